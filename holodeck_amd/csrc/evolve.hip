@@ -1,0 +1,427 @@
+// evolve.hip -- binary evolution kernels of the SAM hot path (FP64).
+//
+//   hd_find_2pwl_norm      <- sam_cyutils.find_2pwl_hardening_norm   (sam_cyutils.pyx:289-398)
+//   hd_hard_func_2pwl_gw   <- sam_cyutils.hard_func_2pwl_gw          (sam_cyutils.pyx:256-286)
+//   hd_dbn_gw              <- _dynamic_binary_number_at_fobs_gw      (sam_cyutils.pyx:784-899)
+//   hd_dbn_2pwl            <- _dynamic_binary_number_at_fobs_2pwl    (sam_cyutils.pyx:506-781)
+//   hd_integrate_3dx1d     <- integrate_differential_number_3dx1d    (sam_cyutils.pyx:115-216)
+//
+// All arithmetic is FP64: the z<->t inversion through the age table and the
+// broken-power-law lifetime integral lose stability in FP32.
+#include "common.hpp"
+#include "../../include/holohip.h"
+#include <math.h>
+
+namespace hd {
+
+// ---------------------------------------------------------------------------
+// Brent root of the 2PL lifetime, one wavefront per (M,Q) element.
+//
+// The 300-step trapezoid lifetime (sam_cyutils.pyx:356-398) is spread over the
+// 64 lanes in contiguous chunks; each lane reproduces the reference's
+// sequential `sepa_log10 -= dx` chain for its chunk, sums its dt's in order and
+// the chunk sums are combined with a butterfly.  The Brent iteration itself
+// (scalar control flow, identical on all lanes) restates the published
+// bracketing algorithm used by scipy's brentq, which the reference calls with
+// xtol=1e-3, rtol=1e-5, maxiter=100 on [-20, 20] (sam_cyutils.pyx:332-349).
+// ---------------------------------------------------------------------------
+struct LifeArgs {
+    double mt, mr, target, rchar, g_in, g_out;
+    double x_lane0;   // sepa_log10 before this lane's first step
+    double dx;
+    int s_begin, s_end;  // this lane's steps [s_begin, s_end), 0-based
+};
+
+__device__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ double lifetime_wave(const EvoConsts& K, const LifeArgs& a, double norm_log10) {
+    double norm = pow(10.0, norm_log10);
+    double part = 0.0;
+    if (a.s_begin < a.s_end) {
+        double x = a.x_lane0;
+        double sepa_left = pow(10.0, x);
+        double dadt_left = hard_func_2pwl_gw(K, a.mt, a.mr, sepa_left, norm, a.rchar, a.g_in, a.g_out);
+        for (int s = a.s_begin; s < a.s_end; s++) {
+            x -= a.dx;
+            double sepa_right = pow(10.0, x);
+            double dadt_right = hard_func_2pwl_gw(K, a.mt, a.mr, sepa_right, norm, a.rchar, a.g_in, a.g_out);
+            double dt = 2.0 * (sepa_right - sepa_left) / (dadt_left + dadt_right);
+            part += dt;
+            sepa_left = sepa_right;
+            dadt_left = dadt_right;
+        }
+    }
+    return wave_sum(part) - a.target;
+}
+
+__global__ void __launch_bounds__(256) k_find_2pwl_norm(EvoConsts K, double target_time, const double* __restrict__ mtot,
+                                                        const double* __restrict__ mrat, int64_t n, double sepa_init,
+                                                        double rchar, double g_in, double g_out, int nsteps,
+                                                        double* __restrict__ norm_log10, double* __restrict__ norm_out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (item >= n) return;   // uniform per wave
+
+    LifeArgs a;
+    a.mt = mtot[item];
+    a.mr = mrat[item];
+    a.target = target_time;
+    a.rchar = rchar;
+    a.g_in = g_in;
+    a.g_out = g_out;
+    double risco_log10 = log10(3.0 * MY_SCHW * a.mt);
+    double x0 = log10(sepa_init);
+    a.dx = (x0 - risco_log10) / nsteps;
+    int chunk = (nsteps + 63) / 64;
+    a.s_begin = min(lane * chunk, nsteps);
+    a.s_end = min(a.s_begin + chunk, nsteps);
+    double x = x0;
+    for (int s = 0; s < a.s_begin; s++) x -= a.dx;
+    a.x_lane0 = x;
+
+    const double xtol = 1e-3, rtol = 1e-5;
+    const int maxiter = 100;
+    double xpre = -20.0, xcur = 20.0, xblk = 0.0, fblk = 0.0, spre = 0.0, scur = 0.0;
+    double fpre = lifetime_wave(K, a, xpre);
+    double fcur = lifetime_wave(K, a, xcur);
+    double res = xcur;
+    bool done = false;
+    if (fpre == 0) { res = xpre; done = true; }
+    else if (fcur == 0) { res = xcur; done = true; }
+    else if (signbit(fpre) == signbit(fcur)) { res = 0.0; done = true; }
+    for (int it = 0; it < maxiter && !done; it++) {
+        if (fpre != 0 && fcur != 0 && (signbit(fpre) != signbit(fcur))) {
+            xblk = xpre;
+            fblk = fpre;
+            spre = scur = xcur - xpre;
+        }
+        if (fabs(fblk) < fabs(fcur)) {
+            xpre = xcur; xcur = xblk; xblk = xpre;
+            fpre = fcur; fcur = fblk; fblk = fpre;
+        }
+        double delta = (xtol + rtol * fabs(xcur)) / 2;
+        double sbis = (xblk - xcur) / 2;
+        if (fcur == 0 || fabs(sbis) < delta) { res = xcur; done = true; break; }
+        if (fabs(spre) > delta && fabs(fcur) < fabs(fpre)) {
+            double stry;
+            if (xpre == xblk) {
+                stry = -fcur * (xcur - xpre) / (fcur - fpre);
+            } else {
+                double dpre = (fpre - fcur) / (xpre - xcur);
+                double dblk = (fblk - fcur) / (xblk - xcur);
+                stry = -fcur * (fblk * dblk - fpre * dpre) / (dblk * dpre * (fblk - fpre));
+            }
+            if (2 * fabs(stry) < fmin(fabs(spre), 3 * fabs(sbis) - delta)) {
+                spre = scur;
+                scur = stry;
+            } else {
+                spre = sbis;
+                scur = sbis;
+            }
+        } else {
+            spre = sbis;
+            scur = sbis;
+        }
+        xpre = xcur;
+        fpre = fcur;
+        if (fabs(scur) > delta) xcur += scur;
+        else xcur += (sbis > 0 ? delta : -delta);
+        fcur = lifetime_wave(K, a, xcur);
+        res = xcur;
+    }
+    if (lane == 0) {
+        norm_log10[item] = res;
+        if (norm_out) norm_out[item] = pow(10.0, res);
+    }
+}
+
+__global__ void k_hard_func_2pwl_gw(EvoConsts K, const double* mtot, const double* mrat, const double* sepa,
+                                    const double* norm, int64_t n, double rchar, double g_in, double g_out,
+                                    double* dadt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dadt[i] = hard_func_2pwl_gw(K, mtot[i], mrat[i], sepa[i], norm[i], rchar, g_in, g_out);
+}
+
+// ---------------------------------------------------------------------------
+// GW-only dynamic binary number: one thread per (M,Q,Z) corner, looping F.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_dbn_gw(EvoConsts K, const double* __restrict__ fobs, int F,
+                                                const double* __restrict__ nden, const double* __restrict__ mtot, int M,
+                                                const double* __restrict__ mrat, int Q, int Z,
+                                                const double* __restrict__ redz_prime,
+                                                const double* __restrict__ gz, const double* __restrict__ gdc, int n_interp,
+                                                double* __restrict__ redz_final, double* __restrict__ diff_num) {
+    int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= (int64_t)M * Q * Z) return;
+    int ii = (int)(cell / ((int64_t)Q * Z));
+    int jj = (int)((cell / Z) % Q);
+    double* rf = redz_final + cell * F;
+    double* dn = diff_num + cell * F;
+    for (int ff = 0; ff < F; ff++) {
+        rf[ff] = -1.0;
+        dn[ff] = 0.0;
+    }
+    double rzp = redz_prime[cell];
+    if (rzp <= 0.0) return;
+    double mt = mtot[ii], mr = mrat[jj];
+    double rad_isco = 3.0 * MY_SCHW * mt;
+    double frst_isco = kepler_freq_from_sepa(K, mt, rad_isco);
+    int idx = ww_decreasing(0, n_interp, rzp, gz);
+    double dcom = interp_at_index(idx, rzp, gz, gdc);
+    double dmpc = dcom / MY_MPC;
+    double cosmo_fact = K.four_pi_c_mpc * (1.0 + rzp) * (dmpc * dmpc);
+    double nd = nden[cell];
+    for (int ff = 0; ff < F; ff++) {
+        rf[ff] = rzp;
+        double ft = fobs[ff];
+        double frst = ft * (1.0 + rzp);
+        if (frst > frst_isco) {
+            for (int fp = ff + 1; fp < F; fp++) rf[fp] = rzp;
+            break;
+        }
+        double sepa = kepler_sepa_from_freq(K, mt, frst);
+        double dadt = hard_gw(K, mt, mr, sepa);
+        double tres = -(2.0 / 3.0) * sepa / dadt;
+        dn[ff] = nd * tres * cosmo_fact;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2PL dynamic binary number.
+// One workgroup per (M,Q) pair.  The workgroup first tabulates the evolution
+// steps in LDS (separation chain, rest-frame frequency, dt, cumulative time;
+// the chain and the cumulative sum are sequential exactly as in the
+// reference), then each thread owns one redshift corner kk and walks the
+// steps, bracketing the target frequencies.  A thread owns its whole output
+// row, so the reference's last-write-wins order per (kk, ff) is preserved.
+// ---------------------------------------------------------------------------
+constexpr int DBN_MAX_STEPS = 512;
+constexpr int DBN_MAX_TAB = 512;
+
+__global__ void __launch_bounds__(128) k_dbn_2pwl(EvoConsts K, const double* __restrict__ fobs, int F, int sorted_f,
+                                                  double sepa_init, int nsteps, const double* __restrict__ hard_norm,
+                                                  double rchar, double g_in, double g_out,
+                                                  const double* __restrict__ nden, const double* __restrict__ mtot, int M,
+                                                  const double* __restrict__ mrat, int Q, const double* __restrict__ redz, int Z,
+                                                  const double* __restrict__ gmt_time,
+                                                  const double* __restrict__ gz_g, const double* __restrict__ gdc_g,
+                                                  const double* __restrict__ gage_g, int n_interp,
+                                                  double* __restrict__ redz_final, double* __restrict__ diff_num) {
+    __shared__ double s_frst[DBN_MAX_STEPS + 1];
+    __shared__ double s_dadt[DBN_MAX_STEPS + 1];
+    __shared__ double s_sepa[DBN_MAX_STEPS + 1];
+    __shared__ double s_dt[DBN_MAX_STEPS + 1];
+    __shared__ double s_tevo[DBN_MAX_STEPS + 1];
+    __shared__ double s_gz[DBN_MAX_TAB], s_gdc[DBN_MAX_TAB], s_gage[DBN_MAX_TAB];
+
+    const int ii = blockIdx.x / Q;
+    const int jj = blockIdx.x % Q;
+    const int tid = threadIdx.x;
+    const double mt = mtot[ii], mr = mrat[jj];
+    const double norm = hard_norm[(int64_t)ii * Q + jj];
+
+    for (int i = tid; i < n_interp; i += blockDim.x) {
+        s_gz[i] = gz_g[i];
+        s_gdc[i] = gdc_g[i];
+        s_gage[i] = gage_g[i];
+    }
+    // separation chain (sequential, as sam_cyutils.pyx:611-637)
+    if (tid == 0) {
+        double x = log10(sepa_init);
+        double risco = 3.0 * MY_SCHW * mt;
+        double dx = (x - log10(risco)) / nsteps;
+        s_sepa[0] = pow(10.0, x);
+        for (int s = 1; s <= nsteps; s++) {
+            x -= dx;
+            s_sepa[s] = x;   // log10 for now
+        }
+    }
+    __syncthreads();
+    for (int s = tid; s <= nsteps; s += blockDim.x) {
+        double sp = (s == 0) ? s_sepa[0] : pow(10.0, s_sepa[s]);
+        s_frst[s] = kepler_freq_from_sepa(K, mt, sp);
+        s_dadt[s] = hard_func_2pwl_gw(K, mt, mr, sp, norm, rchar, g_in, g_out);
+        s_sepa[s] = sp;   // each s is read and written by the same thread
+    }
+    __syncthreads();
+    for (int s = tid + 1; s <= nsteps; s += blockDim.x)
+        s_dt[s] = 2.0 * (s_sepa[s] - s_sepa[s - 1]) / (s_dadt[s - 1] + s_dadt[s]);
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        s_tevo[0] = 0.0;
+        for (int s = 1; s <= nsteps; s++) {
+            t += s_dt[s];
+            s_tevo[s] = t;
+        }
+    }
+    __syncthreads();
+
+    const double age_universe = s_gage[n_interp - 1];
+    for (int kk = tid; kk < Z; kk += blockDim.x) {
+        const int64_t cell = ((int64_t)ii * Q + jj) * Z + kk;
+        double* rf = redz_final + cell * F;
+        double* dn = diff_num + cell * F;
+        for (int ff = 0; ff < F; ff++) {
+            rf[ff] = -1.0;
+            dn[ff] = 0.0;
+        }
+        // age at this redshift edge from the table (sam_cyutils.pyx:586-601)
+        const double zk = redz[kk];
+        int ia = 0;
+        while ((ia + 1 < n_interp) && (s_gz[ia + 1] > zk)) ia++;
+        if (ia > n_interp - 2) ia = n_interp - 2;
+        const double age_k = interp_at_index(ia, zk, s_gz, s_gage);
+        const double gmt = gmt_time[cell];
+        const double nd = nden[cell];
+
+        int idx_left = 0;
+        for (int step = 0; step < nsteps; step++) {
+            const double dt = s_dt[step + 1];
+            const double time_right = s_tevo[step + 1] + gmt + age_k;
+            const double time_left = time_right - dt;
+            if (time_left > age_universe) continue;
+            idx_left = ww_increasing(idx_left, n_interp, time_left, s_gage);
+            const double redz_left = interp_at_index(idx_left, time_left, s_gage, s_gz);
+            if (redz_left < 0.0) continue;
+            const int idx_right = ww_increasing(idx_left, n_interp, time_right, s_gage);
+            double redz_right = interp_at_index(idx_right, time_right, s_gage, s_gz);
+            if (redz_right < 0.0) redz_right = 0.0;
+            const double fl = s_frst[step] / (1.0 + redz_left);
+            const double fr = s_frst[step + 1] / (1.0 + redz_right);
+
+            int ff = 0;
+            if (sorted_f) {
+                // first target >= fl (targets are non-decreasing)
+                int lo = 0, hi = F;
+                while (lo < hi) {
+                    int mid = (lo + hi) >> 1;
+                    if (fobs[mid] < fl) lo = mid + 1;
+                    else hi = mid;
+                }
+                ff = lo;
+            }
+            for (; ff < F; ff++) {
+                const double ft = fobs[ff];
+                if ((ft < fl) || (fr < ft)) {
+                    if (sorted_f) break;   // past the contiguous bracketed range
+                    continue;
+                }
+                const double new_time = interp_between(ft, fl, fr, time_left, time_right);
+                if (new_time > s_gage[n_interp - 1]) break;
+                const int inew = ww_increasing(idx_left, n_interp, new_time, s_gage);
+                const double new_redz = interp_at_index(inew, new_time, s_gage, s_gz);
+                const double dcom = interp_at_index(inew, new_time, s_gage, s_gdc);
+                rf[ff] = new_redz;
+                const double frst = ft * (1.0 + new_redz);
+                const double sepa = kepler_sepa_from_freq(K, mt, frst);
+                const double dadt = hard_func_2pwl_gw(K, mt, mr, sepa, norm, rchar, g_in, g_out);
+                const double tres = -(2.0 / 3.0) * sepa / dadt;
+                const double dmpc = dcom / MY_MPC;
+                const double cosmo_fact = K.four_pi_c_mpc * (1.0 + new_redz) * (dmpc * dmpc);
+                dn[ff] = nd * tres * cosmo_fact;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3D trapezoid x 1D Riemann integration (sam_cyutils.pyx:166-216).
+// One thread per output (m,q,z,f); corner order and operation order follow
+// the reference loop.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_integrate_3dx1d(const double* __restrict__ lm, int M, const double* __restrict__ mr,
+                                                         int Q, const double* __restrict__ rz, int Z,
+                                                         const double* __restrict__ dlnf, int F,
+                                                         const double* __restrict__ dnum, double* __restrict__ numb) {
+    int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nout = (int64_t)(M - 1) * (Q - 1) * (Z - 1) * F;
+    if (o >= nout) return;
+    int ff = (int)(o % F);
+    int64_t t = o / F;
+    int zz = (int)(t % (Z - 1));
+    t /= (Z - 1);
+    int qq = (int)(t % (Q - 1));
+    int mm = (int)(t / (Q - 1));
+    double dm = lm[mm + 1] - lm[mm];
+    double dmdq = dm * (mr[qq + 1] - mr[qq]);
+    double dmdqdz = dmdq * (rz[zz + 1] - rz[zz]);
+    double temp = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                temp += dnum[((((int64_t)(mm + i) * Q) + (qq + j)) * Z + (zz + k)) * F + ff];
+    numb[o] = temp * dmdqdz * dlnf[ff] / 8.0;
+}
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" int hd_find_2pwl_norm(double target_time, const double* mtot, const double* mrat, int64_t n,
+                                 double sepa_init, double rchar, double gamma_inner, double gamma_outer, int nsteps,
+                                 double* norm_log10, double* norm, void* stream) {
+    if (n < 0 || nsteps < 1 || !mtot || !mrat || !norm_log10) return set_error(HD_ERR_ARG, "hd_find_2pwl_norm: bad arguments");
+    if (n == 0) return HD_OK;
+    const int waves_per_block = 4;
+    dim3 grid((unsigned)cdiv(n, waves_per_block));
+    hipLaunchKernelGGL(k_find_2pwl_norm, grid, dim3(64 * waves_per_block), 0, (hipStream_t)stream, evo_consts(),
+                       target_time, mtot, mrat, n, sepa_init, rchar, gamma_inner, gamma_outer, nsteps, norm_log10, norm);
+    return check_launch("k_find_2pwl_norm");
+}
+
+extern "C" int hd_hard_func_2pwl_gw(const double* mtot, const double* mrat, const double* sepa, const double* norm,
+                                    int64_t n, double rchar, double gamma_inner, double gamma_outer, double* dadt,
+                                    void* stream) {
+    if (n < 0 || (n > 0 && (!mtot || !mrat || !sepa || !norm || !dadt)))
+        return set_error(HD_ERR_ARG, "hd_hard_func_2pwl_gw: bad arguments");
+    if (n == 0) return HD_OK;
+    hipLaunchKernelGGL(k_hard_func_2pwl_gw, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       evo_consts(), mtot, mrat, sepa, norm, n, rchar, gamma_inner, gamma_outer, dadt);
+    return check_launch("k_hard_func_2pwl_gw");
+}
+
+extern "C" int hd_dbn_gw(const double* fobs_orb, int F, const double* nden, const double* mtot, int M,
+                         const double* mrat, int Q, int Z, const double* redz_prime, const double* grid_z,
+                         const double* grid_dcom, int n_interp, double* redz_final, double* diff_num, void* stream) {
+    if (F < 1 || M < 1 || Q < 1 || Z < 1 || n_interp < 2)
+        return set_error(HD_ERR_ARG, "hd_dbn_gw: bad sizes F=%d M=%d Q=%d Z=%d n_interp=%d", F, M, Q, Z, n_interp);
+    int64_t n = (int64_t)M * Q * Z;
+    hipLaunchKernelGGL(k_dbn_gw, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, evo_consts(), fobs_orb,
+                       F, nden, mtot, M, mrat, Q, Z, redz_prime, grid_z, grid_dcom, n_interp, redz_final, diff_num);
+    return check_launch("k_dbn_gw");
+}
+
+extern "C" int hd_dbn_2pwl(const double* fobs_orb, int F, int fobs_sorted, double sepa_init, int num_steps,
+                           const double* hard_norm, double rchar, double gamma_inner, double gamma_outer,
+                           const double* nden, const double* mtot, int M, const double* mrat, int Q, const double* redz,
+                           int Z, const double* gmt_time, const double* grid_z, const double* grid_dcom,
+                           const double* grid_age, int n_interp, double* redz_final, double* diff_num, void* stream) {
+    if (F < 1 || M < 1 || Q < 1 || Z < 1 || n_interp < 2 || num_steps < 1)
+        return set_error(HD_ERR_ARG, "hd_dbn_2pwl: bad sizes F=%d M=%d Q=%d Z=%d n_interp=%d steps=%d", F, M, Q, Z,
+                         n_interp, num_steps);
+    if (num_steps > DBN_MAX_STEPS) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: num_steps=%d > %d", num_steps, DBN_MAX_STEPS);
+    if (n_interp > DBN_MAX_TAB) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: n_interp=%d > %d", n_interp, DBN_MAX_TAB);
+    if ((int64_t)M * Q > 0x7fffffff) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: M*Q too large");
+    hipLaunchKernelGGL(k_dbn_2pwl, dim3(M * Q), dim3(128), 0, (hipStream_t)stream, evo_consts(), fobs_orb, F,
+                       fobs_sorted, sepa_init, num_steps, hard_norm, rchar, gamma_inner, gamma_outer, nden, mtot, M,
+                       mrat, Q, redz, Z, gmt_time, grid_z, grid_dcom, grid_age, n_interp, redz_final, diff_num);
+    return check_launch("k_dbn_2pwl");
+}
+
+extern "C" int hd_integrate_3dx1d(const double* log10_mtot, int M, const double* mrat, int Q, const double* redz, int Z,
+                                  const double* dln_freq, int F, const double* dnum, double* numb, void* stream) {
+    if (M < 2 || Q < 2 || Z < 2 || F < 1) return set_error(HD_ERR_ARG, "hd_integrate_3dx1d: bad sizes");
+    int64_t n = (int64_t)(M - 1) * (Q - 1) * (Z - 1) * F;
+    hipLaunchKernelGGL(k_integrate_3dx1d, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, log10_mtot,
+                       M, mrat, Q, redz, Z, dln_freq, F, dnum, numb);
+    return check_launch("k_integrate_3dx1d");
+}

@@ -1,0 +1,379 @@
+// realize.hip -- discrete realizations of the binary population and the
+// single-source / background split of h_c^2.
+//
+// <- holodeck/cyutils.pyx: _loudest_hc_from_sorted (1266-1344),
+//    _loudest_hc_and_par_from_sorted_redz (1615-1767), _ss_bg_hc (935-1014),
+//    _ss_bg_hc_and_par (1068-1178), _sam_poisson_gwb (862-897).
+//
+// Per (frequency f, realization r) the reference draws a count n_b for every
+// bin b, fills L loud slots from bins in a fixed priority order (with
+// multiplicity) and sums n_b*h_b over the rest.  Here:
+//   * the count is either injected ((B,F,R) array: the exact parity path) or
+//     drawn from a counter-based Philox stream keyed by (seed, b, f, r);
+//   * a workgroup handles one frequency and one chunk of bins, one thread per
+//     realization; bin data (lambda, tail probabilities, h, priority key) are
+//     staged through LDS and shared by all realizations;
+//   * each thread keeps the L best instances of its chunk in registers
+//     (priority = order rank for the sorted walk, (h desc, bin asc) for argmax)
+//     and the FP64 totals sum(n h), sum(n h p_k);
+//   * a merge kernel combines the chunk lists per (f, r); the background is
+//     total - sum(selected h), which is exactly the reference's accounting
+//     (sum(n h) - max for ss_bg_hc; sum((n - taken) h) for the sorted walk,
+//     including its treatment of non-integer Normal draws).
+#include "common.hpp"
+#include "rng.hpp"
+#include "../../include/holohip.h"
+
+namespace hd {
+
+constexpr int LMAX = 16;
+constexpr int NPAR_MAX = 8;
+constexpr int RZ_CHUNK = 16384;   // bins per workgroup
+constexpr int RZ_SUB = 256;       // bins staged in LDS at a time
+constexpr uint64_t KEY_NONE = ~0ull;
+
+struct RealizeArgs {
+    const double* number;    // (B,F) or null for injected counts
+    const double* counts;    // (B,F,R) injected counts or null
+    const double* h2fdf;     // (B,F)
+    const int32_t* rank;     // (B) position in order (SORTED) or null
+    const double* par;       // (npar,B,F) or null
+    int64_t B;
+    int F, R, L, mode, flags, npar, nchunks;
+    double thresh;
+    uint64_t seed;
+    // partials
+    double* p_total;      // (F, nchunks, R)
+    double* p_ptot;       // (npar, F, nchunks, R)
+    uint64_t* p_key;      // (F, nchunks, R, L)
+    int32_t* p_bin;       // (F, nchunks, R, L)
+};
+
+// ascending image of a double, so that larger h -> smaller key when inverted
+__device__ __forceinline__ uint64_t h_priority(double h) {
+    uint64_t u = (uint64_t)__double_as_longlong(h);
+    u = (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+    return ~u;
+}
+
+__device__ __forceinline__ bool better(uint64_t ka, int32_t ba, uint64_t kb, int32_t bb) {
+    return (ka < kb) || (ka == kb && ba < bb);
+}
+
+// insert one instance into a sorted register list of LMAX slots
+__device__ __forceinline__ void list_insert(uint64_t (&lk)[LMAX], int32_t (&lb)[LMAX], uint64_t k, int32_t b) {
+#pragma unroll
+    for (int i = 0; i < LMAX; i++) {
+        bool sw = better(k, b, lk[i], lb[i]);
+        uint64_t tk = lk[i];
+        int32_t tb = lb[i];
+        lk[i] = sw ? k : tk;
+        lb[i] = sw ? b : tb;
+        k = sw ? tk : k;
+        b = sw ? tb : b;
+    }
+}
+
+template <bool INJECTED>
+__global__ void __launch_bounds__(512) k_realize_chunk(RealizeArgs A) {
+    __shared__ double s_lam[RZ_SUB];
+    __shared__ double s_q1[RZ_SUB];
+    __shared__ double s_el[RZ_SUB];
+    __shared__ double s_h[RZ_SUB];
+    __shared__ uint64_t s_key[RZ_SUB];
+    __shared__ double s_par[NPAR_MAX][RZ_SUB];
+
+    const int f = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const int64_t b0 = (int64_t)chunk * RZ_CHUNK;
+    const int64_t b1 = min(b0 + (int64_t)RZ_CHUNK, A.B);
+    const int F = A.F, R = A.R, L = A.L;
+    const bool use_list = (A.mode != HD_SEL_NONE);
+    const bool sorted_mode = (A.mode == HD_SEL_SORTED);
+    const bool skip_zero_h = (A.flags & HD_FLAG_SKIP_ZERO_H) != 0;
+
+    for (int r0 = 0; r0 < R; r0 += blockDim.x) {
+        const int r = r0 + threadIdx.x;
+        const bool active = r < R;
+        uint64_t lk[LMAX];
+        int32_t lb[LMAX];
+#pragma unroll
+        for (int i = 0; i < LMAX; i++) {
+            lk[i] = KEY_NONE;
+            lb[i] = 0x7fffffff;
+        }
+        double total = 0.0;
+        double ptot[NPAR_MAX];
+#pragma unroll
+        for (int k = 0; k < NPAR_MAX; k++) ptot[k] = 0.0;
+
+        for (int64_t s0 = b0; s0 < b1; s0 += RZ_SUB) {
+            const int ns = (int)min((int64_t)RZ_SUB, b1 - s0);
+            __syncthreads();
+            for (int c = threadIdx.x; c < ns; c += blockDim.x) {
+                int64_t b = s0 + c;
+                double h = A.h2fdf[b * F + f];
+                s_h[c] = h;
+                if (!INJECTED) {
+                    double lam = A.number[b * F + f];
+                    s_lam[c] = lam;
+                    s_q1[c] = -expm1(-lam);
+                    s_el[c] = exp(-lam);
+                }
+                s_key[c] = sorted_mode ? (uint64_t)A.rank[b] : h_priority(h);
+                for (int k = 0; k < A.npar; k++) s_par[k][c] = A.par[((int64_t)k * A.B + b) * F + f];
+            }
+            __syncthreads();
+            if (!active) continue;
+            for (int c = 0; c < ns; c++) {
+                const int64_t b = s0 + c;
+                double n;
+                if (INJECTED) {
+                    n = A.counts[(b * F + f) * R + r];
+                } else {
+                    double lam = s_lam[c];
+                    if (!(lam > 0.0)) continue;
+                    Stream st;
+                    st.init((uint32_t)b, (uint32_t)f, (uint32_t)r, 0u, A.seed);
+                    if (lam > A.thresh) {
+                        n = normal_count(st, lam);
+                    } else if (lam < 10.0) {
+                        double u = st.next();
+                        if (!(u < s_q1[c])) continue;   // n = 0: the common case
+                        n = poisson_inversion(u, lam, s_q1[c], s_el[c]);
+                    } else {
+                        n = poisson_ptrs(st, lam);
+                    }
+                }
+                const double h = s_h[c];
+                if (sorted_mode) {
+                    if (n < 1) continue;
+                    if (skip_zero_h && h == 0) continue;
+                } else {
+                    if (n == 0) continue;
+                }
+                const double nh = n * h;
+                total += nh;
+                for (int k = 0; k < A.npar; k++) ptot[k] += nh * s_par[k][c];
+                if (use_list && n > 0) {
+                    if (!sorted_mode && !(h > 0)) continue;   // argmax: strict cur > max >= 0
+                    const uint64_t key = s_key[c];
+                    const int32_t bb = (int32_t)b;
+                    // the same (key, bin) for every instance of this bin
+                    int ninst = (int)min(ceil(n), (double)L);
+                    for (int i = 0; i < ninst; i++) {
+                        // compare against slot L-1 (dynamic index avoided: scan)
+                        uint64_t wk = KEY_NONE;
+                        int32_t wb = 0x7fffffff;
+#pragma unroll
+                        for (int j = 0; j < LMAX; j++)
+                            if (j == L - 1) { wk = lk[j]; wb = lb[j]; }
+                        if (!better(key, bb, wk, wb)) break;
+                        list_insert(lk, lb, key, bb);
+                    }
+                }
+            }
+        }
+        if (active) {
+            int64_t pr = ((int64_t)f * A.nchunks + chunk) * R + r;
+            A.p_total[pr] = total;
+            for (int k = 0; k < A.npar; k++) A.p_ptot[(int64_t)k * F * A.nchunks * R + pr] = ptot[k];
+            if (use_list) {
+#pragma unroll
+                for (int j = 0; j < LMAX; j++)
+                    if (j < L) {
+                        A.p_key[pr * L + j] = lk[j];
+                        A.p_bin[pr * L + j] = lb[j];
+                    }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_realize_merge(RealizeArgs A, double* __restrict__ hc2ss,
+                                                       int64_t* __restrict__ ssidx, double* __restrict__ hc2bg,
+                                                       double* __restrict__ bgpar) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = A.F, R = A.R, L = A.L, NC = A.nchunks;
+    if (t >= (int64_t)F * R) return;
+    const int f = (int)(t / R), r = (int)(t % R);
+    const bool use_list = (A.mode != HD_SEL_NONE);
+    uint64_t lk[LMAX];
+    int32_t lb[LMAX];
+#pragma unroll
+    for (int i = 0; i < LMAX; i++) {
+        lk[i] = KEY_NONE;
+        lb[i] = 0x7fffffff;
+    }
+    double total = 0.0;
+    double ptot[NPAR_MAX];
+#pragma unroll
+    for (int k = 0; k < NPAR_MAX; k++) ptot[k] = 0.0;
+    for (int c = 0; c < NC; c++) {
+        int64_t pr = ((int64_t)f * NC + c) * R + r;
+        total += A.p_total[pr];
+        for (int k = 0; k < A.npar; k++) ptot[k] += A.p_ptot[(int64_t)k * F * NC * R + pr];
+        if (use_list) {
+            for (int j = 0; j < L; j++) {
+                uint64_t k = A.p_key[pr * L + j];
+                int32_t b = A.p_bin[pr * L + j];
+                if (k == KEY_NONE) break;
+                uint64_t wk = KEY_NONE;
+                int32_t wb = 0x7fffffff;
+#pragma unroll
+                for (int q = 0; q < LMAX; q++)
+                    if (q == L - 1) { wk = lk[q]; wb = lb[q]; }
+                if (!better(k, b, wk, wb)) break;
+                list_insert(lk, lb, k, b);
+            }
+        }
+    }
+    double sel = 0.0;
+    double psel[NPAR_MAX];
+#pragma unroll
+    for (int k = 0; k < NPAR_MAX; k++) psel[k] = 0.0;
+    if (use_list) {
+#pragma unroll
+        for (int j = 0; j < LMAX; j++) {
+            if (j >= L) continue;
+            int64_t o = ((int64_t)f * R + r) * L + j;
+            if (lk[j] == KEY_NONE) {
+                hc2ss[o] = 0.0;
+                ssidx[o] = -1;
+            } else {
+                int64_t b = lb[j];
+                double h = A.h2fdf[b * F + f];
+                hc2ss[o] = h;
+                ssidx[o] = b;
+                sel += h;
+                for (int k = 0; k < A.npar; k++) psel[k] += h * A.par[((int64_t)k * A.B + b) * F + f];
+            }
+        }
+    }
+    double bg = total - sel;
+    hc2bg[(int64_t)f * R + r] = bg;
+    if (bgpar) {
+        bool none = use_list && (lk[0] == KEY_NONE) && (A.mode == HD_SEL_ARGMAX);
+        for (int k = 0; k < A.npar; k++) {
+            double v = none ? __builtin_nan("") : (ptot[k] - psel[k]) / bg;
+            bgpar[((int64_t)k * F + f) * R + r] = v;
+        }
+    }
+}
+
+__global__ void k_rank_from_order(const int64_t* __restrict__ order, int64_t B, int32_t* __restrict__ rank) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= B) return;
+    rank[order[p]] = (int32_t)p;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t realize_ws(int64_t B, int F, int R, int L, int npar) {
+    int64_t nch = cdiv(B, RZ_CHUNK);
+    size_t pr = (size_t)F * nch * R;
+    size_t s = 0;
+    s += align256((size_t)B * 4);               // rank
+    s += align256(pr * 8);                      // totals
+    s += align256(pr * 8 * (size_t)npar);       // param totals
+    s += align256(pr * 8 * (size_t)L);          // keys
+    s += align256(pr * 4 * (size_t)L);          // bins
+    return s + 256;
+}
+
+static int realize_impl(bool injected, const double* number, const double* counts, const double* h2fdf,
+                        const int64_t* order, int64_t B, int F, int R, int L, int mode, int flags, double thresh,
+                        uint64_t seed, const double* par, int npar, double* hc2ss, int64_t* ssidx, double* hc2bg,
+                        double* bgpar, void* ws, size_t ws_bytes, hipStream_t st) {
+    if (B < 1 || F < 1 || R < 1 || L < 0 || !h2fdf || !hc2bg) return set_error(HD_ERR_ARG, "realize: bad sizes/pointers");
+    if (B > 0x7fffffffll) return set_error(HD_ERR_SIZE, "realize: B=%lld exceeds 2^31-1", (long long)B);
+    if (F > 65535) return set_error(HD_ERR_SIZE, "realize: F=%d > 65535", F);
+    if (mode != HD_SEL_SORTED && mode != HD_SEL_ARGMAX && mode != HD_SEL_NONE)
+        return set_error(HD_ERR_ARG, "realize: unknown mode %d", mode);
+    if (mode == HD_SEL_NONE) L = 0;
+    else if (L < 1) return set_error(HD_ERR_ARG, "realize: L must be >= 1 for loud-source modes");
+    if (L > LMAX) return set_error(HD_ERR_SIZE, "realize: L=%d > %d", L, LMAX);
+    if (npar < 0 || npar > NPAR_MAX || (npar > 0 && !par)) return set_error(HD_ERR_ARG, "realize: npar=%d invalid", npar);
+    if (mode == HD_SEL_SORTED && !order) return set_error(HD_ERR_ARG, "realize: sorted mode needs an order");
+    if (mode != HD_SEL_NONE && (!hc2ss || !ssidx)) return set_error(HD_ERR_ARG, "realize: hc2ss/ssidx required");
+    if (injected && !counts) return set_error(HD_ERR_ARG, "realize: counts required");
+    if (!injected && !number) return set_error(HD_ERR_ARG, "realize: number required");
+    size_t need = realize_ws(B, F, R, L, npar);
+    if (!ws || ws_bytes < need) return set_error(HD_ERR_SIZE, "realize: workspace too small (%zu < %zu)", ws_bytes, need);
+
+    RealizeArgs A;
+    A.number = number;
+    A.counts = counts;
+    A.h2fdf = h2fdf;
+    A.par = par;
+    A.B = B;
+    A.F = F;
+    A.R = R;
+    A.L = L;
+    A.mode = mode;
+    A.flags = flags;
+    A.npar = npar;
+    A.thresh = thresh;
+    A.seed = seed;
+    A.nchunks = (int)cdiv(B, RZ_CHUNK);
+    size_t pr = (size_t)F * A.nchunks * R;
+    char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+    int32_t* rank = (int32_t*)p;  p += align256((size_t)B * 4);
+    A.p_total = (double*)p;       p += align256(pr * 8);
+    A.p_ptot = (double*)p;        p += align256(pr * 8 * (size_t)npar);
+    A.p_key = (uint64_t*)p;       p += align256(pr * 8 * (size_t)L);
+    A.p_bin = (int32_t*)p;
+    A.rank = (mode == HD_SEL_SORTED) ? rank : nullptr;
+
+    if (mode == HD_SEL_SORTED) {
+        hipLaunchKernelGGL(k_rank_from_order, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, st, order, B, rank);
+        int rc = check_launch("k_rank_from_order");
+        if (rc) return rc;
+    }
+    int threads = (int)min((int64_t)512, cdiv(R, 64) * 64);
+    dim3 grid(A.nchunks, F);
+    if (injected) hipLaunchKernelGGL(k_realize_chunk<true>, grid, dim3(threads), 0, st, A);
+    else hipLaunchKernelGGL(k_realize_chunk<false>, grid, dim3(threads), 0, st, A);
+    int rc = check_launch("k_realize_chunk");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_realize_merge, dim3((unsigned)cdiv((int64_t)F * R, 256)), dim3(256), 0, st, A, hc2ss, ssidx,
+                       hc2bg, (npar > 0) ? bgpar : nullptr);
+    return check_launch("k_realize_merge");
+}
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" size_t hd_realize_counts_workspace_size(int64_t B, int F, int R, int L, int npar) {
+    return realize_ws(B, F, R, L, npar);
+}
+
+extern "C" int hd_realize_counts(const double* counts, const double* h2fdf, const int64_t* order, int64_t B, int F, int R,
+                                 int L, int mode, int flags, const double* par, int npar, double* hc2ss,
+                                 int64_t* ssidx, double* hc2bg, double* bgpar, void* workspace, size_t ws_bytes,
+                                 void* stream) {
+    return realize_impl(true, nullptr, counts, h2fdf, order, B, F, R, L, mode, flags, 0.0, 0, par, npar, hc2ss, ssidx,
+                        hc2bg, bgpar, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" size_t hd_realize_workspace_size(int64_t B, int F, int R, int L, int npar) {
+    return realize_ws(B, F, R, L, npar);
+}
+
+extern "C" int hd_realize(const double* number, const double* h2fdf, const int64_t* order, int64_t B, int F, int R, int L,
+                          int mode, int flags, double normal_threshold, uint64_t seed, const double* par, int npar,
+                          double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar, void* workspace,
+                          size_t ws_bytes, void* stream) {
+    return realize_impl(false, number, nullptr, h2fdf, order, B, F, R, L, mode, flags, normal_threshold, seed, par, npar,
+                        hc2ss, ssidx, hc2bg, bgpar, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" void hd_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    U4 c = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    U4 o = philox4x32_10(c, key[0], key[1]);
+    out[0] = o.x;
+    out[1] = o.y;
+    out[2] = o.z;
+    out[3] = o.w;
+}

@@ -1,0 +1,128 @@
+"""Device-level pipeline of the SAM GWB hot path.
+
+Every function takes and returns device tensors (float64 unless stated) and
+issues its work on the current stream through the C-ABI of libholohip.so;
+nothing here computes per-cell values on the host.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as D
+from .constants import NWTG, SPLC
+
+#: 8 G^(5/3) pi^(2/3) / sqrt(10) / c^4 with astropy G, c (utils.py:40)
+GW_SRC_CONST = 8 * np.power(NWTG, 5 / 3) * np.power(np.pi, 2 / 3) / np.sqrt(10) / np.power(SPLC, 4)
+NORMAL_THRESHOLD = 1e10
+
+
+def find_2pwl_norm(target_time, mtot, mrat, sepa_init, rchar, gamma_inner, gamma_outer, nsteps):
+    """(log10 norm, norm) per element (sam_cyutils.pyx:289-353, hardening.py:1413-1416)."""
+    mt = D.to_dev(mtot)
+    mr = D.to_dev(mrat)
+    out = D.empty(mt.numel())
+    norm = D.empty(mt.numel())
+    _lib.call("hd_find_2pwl_norm", float(target_time), D.ptr(mt), D.ptr(mr), mt.numel(), float(sepa_init),
+              float(rchar), float(gamma_inner), float(gamma_outer), int(nsteps), D.ptr(out), D.ptr(norm),
+              D.stream_ptr())
+    return out, norm
+
+
+def static_density(mtot_d, mrat_d, redz_d, par, cosmo):
+    """(dens, zprime, gmt_time) before scatter and stalled-zeroing, all (M,Q,Z)."""
+    M, Q, Z = mtot_d.numel(), mrat_d.numel(), redz_d.numel()
+    tabs = cosmo.device_tables()
+    dens = D.empty((M, Q, Z))
+    zp = D.empty((M, Q, Z))
+    gmt = D.empty((M, Q, Z))
+    par = np.ascontiguousarray(par, dtype=np.float64)
+    assert par.size == _lib.HD_DENS_NPAR
+    _lib.call("hd_static_density", D.ptr(mtot_d), M, D.ptr(mrat_d), Q, D.ptr(redz_d), Z, par.ctypes.data,
+              D.ptr(tabs["dc_tab"]), D.ptr(tabs["dc_der"]), tabs["n_tab"], D.ptr(dens), D.ptr(zp), D.ptr(gmt),
+              D.stream_ptr())
+    return dens, zp, gmt
+
+
+def zero_stalled(dens, zprime):
+    _lib.call("hd_zero_stalled", D.ptr(dens), D.ptr(zprime), dens.numel(), D.stream_ptr())
+
+
+def dbn_gw(fobs_orb_d, nden, mtot_d, mrat_d, redz_prime, cosmo):
+    M, Q, Z = nden.shape
+    F = fobs_orb_d.numel()
+    tabs = cosmo.device_tables()
+    rf = D.empty((M, Q, Z, F))
+    dn = D.empty((M, Q, Z, F))
+    _lib.call("hd_dbn_gw", D.ptr(fobs_orb_d), F, D.ptr(nden), D.ptr(mtot_d), M, D.ptr(mrat_d), Q, Z,
+              D.ptr(redz_prime), D.ptr(tabs["grid_z"]), D.ptr(tabs["grid_dcom"]), cosmo.size, D.ptr(rf), D.ptr(dn),
+              D.stream_ptr())
+    return rf, dn
+
+
+def dbn_2pwl(fobs_orb_d, fobs_sorted, sepa_init, num_steps, norm_d, rchar, gamma_inner, gamma_outer, nden, mtot_d,
+             mrat_d, redz_d, gmt_time, cosmo):
+    M, Q, Z = nden.shape
+    F = fobs_orb_d.numel()
+    tabs = cosmo.device_tables()
+    rf = D.empty((M, Q, Z, F))
+    dn = D.empty((M, Q, Z, F))
+    _lib.call("hd_dbn_2pwl", D.ptr(fobs_orb_d), F, int(bool(fobs_sorted)), float(sepa_init), int(num_steps),
+              D.ptr(norm_d), float(rchar), float(gamma_inner), float(gamma_outer), D.ptr(nden), D.ptr(mtot_d), M,
+              D.ptr(mrat_d), Q, D.ptr(redz_d), Z, D.ptr(gmt_time), D.ptr(tabs["grid_z"]), D.ptr(tabs["grid_dcom"]),
+              D.ptr(tabs["grid_age"]), cosmo.size, D.ptr(rf), D.ptr(dn), D.stream_ptr())
+    return rf, dn
+
+
+def integrate_3dx1d(log10_mtot_d, mrat_d, redz_d, dlnf_d, dnum):
+    M, Q, Z, F = dnum.shape
+    out = D.empty((M - 1, Q - 1, Z - 1, F))
+    _lib.call("hd_integrate_3dx1d", D.ptr(log10_mtot_d), M, D.ptr(mrat_d), Q, D.ptr(redz_d), Z, D.ptr(dlnf_d), F,
+              D.ptr(dnum), D.ptr(out), D.stream_ptr())
+    return out
+
+
+def h2fdf_redz(mt_cent_d, mr_cent_d, fc_d, fcdf_d, redz_final, cosmo):
+    M, Q, Z, F = redz_final.shape
+    tabs = cosmo.device_tables()
+    out = D.empty((M - 1, Q - 1, Z - 1, F))
+    _lib.call("hd_h2fdf_redz", D.ptr(mt_cent_d), M, D.ptr(mr_cent_d), Q, Z, D.ptr(fc_d), D.ptr(fcdf_d), F,
+              D.ptr(redz_final), float(GW_SRC_CONST), cosmo.Om0, cosmo.h, D.ptr(tabs["dc_tab"]),
+              D.ptr(tabs["dc_der"]), tabs["n_tab"], D.ptr(out), D.stream_ptr())
+    return out
+
+
+def order_desc(key2d, col=0):
+    """Stable descending order of rows of key2d (B, stride) by column `col` (int64 device tensor)."""
+    B, stride = key2d.shape
+    order = D.empty(B, dtype=torch.int64)
+    ws = D.workspace(_lib.size("hd_order_workspace_size", B))
+    _lib.call("hd_order_desc", D.ptr(key2d), B, stride, int(col), D.ptr(order), D.ptr(ws), ws.numel(),
+              D.stream_ptr())
+    return order
+
+
+def realize(number2d, h2fdf2d, nreals, nloud, mode, order=None, counts=None, flags=0, par=None, seed=0,
+            normal_threshold=NORMAL_THRESHOLD):
+    """Realizations over (B,F) bins.  Returns (hc2ss (F,R,L), ssidx (F,R,L), hc2bg (F,R), bgpar (npar,F,R)|None)."""
+    B, F = h2fdf2d.shape
+    R = int(nreals)
+    L = 0 if mode == _lib.HD_SEL_NONE else int(nloud)
+    npar = 0 if par is None else int(par.shape[0])
+    hc2bg = D.empty((F, R))
+    hc2ss = D.empty((F, R, max(L, 1)))
+    ssidx = D.empty((F, R, max(L, 1)), dtype=torch.int64)
+    bgpar = D.empty((npar, F, R)) if npar else None
+    if counts is not None:
+        ws = D.workspace(_lib.size("hd_realize_counts_workspace_size", B, F, R, L, npar))
+        _lib.call("hd_realize_counts", D.ptr(counts), D.ptr(h2fdf2d), D.ptr(order), B, F, R, L, mode, flags,
+                  D.ptr(par), npar, D.ptr(hc2ss), D.ptr(ssidx), D.ptr(hc2bg), D.ptr(bgpar), D.ptr(ws), ws.numel(),
+                  D.stream_ptr())
+    else:
+        ws = D.workspace(_lib.size("hd_realize_workspace_size", B, F, R, L, npar))
+        _lib.call("hd_realize", D.ptr(number2d), D.ptr(h2fdf2d), D.ptr(order), B, F, R, L, mode, flags,
+                  float(normal_threshold), int(seed) & 0xFFFFFFFFFFFFFFFF, D.ptr(par), npar, D.ptr(hc2ss),
+                  D.ptr(ssidx), D.ptr(hc2bg), D.ptr(bgpar), D.ptr(ws), ws.numel(), D.stream_ptr())
+    if L == 0:
+        hc2ss = hc2ss[..., :0]
+        ssidx = ssidx[..., :0]
+    return hc2ss, ssidx, hc2bg, bgpar

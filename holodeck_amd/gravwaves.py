@@ -1,0 +1,42 @@
+"""GW strain of a binned population (holodeck/gravwaves.py:470-542, 694-725), device compute."""
+import numpy as np
+
+from . import _lib
+from . import device as D
+from . import engine
+from . import utils
+
+
+def char_strain_sq_from_bin_edges_redz(edges, redz, cosmo=None):
+    """h_s^2 f/df per bin: edges [mtot, mrat, redz, fobs_orb_edges], redz (M,Q,Z,F) -> (M-1,Q-1,Z-1,F)."""
+    from .cosmology import cosmo as _c
+    cosmo = _c if cosmo is None else cosmo
+    assert len(edges) == 4
+    fo = np.asarray(edges[-1], dtype=float)
+    fc = utils.midpoints(fo)
+    df = np.diff(fo)
+    rf = D.to_dev(redz)
+    out = engine.h2fdf_redz(D.to_dev(utils.midpoints(edges[0])), D.to_dev(utils.midpoints(edges[1])),
+                            D.to_dev(fc), D.to_dev(fc / df), rf, cosmo)
+    return D.to_host(out)
+
+
+def _gws_from_number_grid_integrated_redz(edges, redz, number, realize, sum=True, *, seed=None, cosmo=None):
+    """h_c of the GWB without a single-source split (gravwaves.py:470-542); realize int -> (F, R)."""
+    from .cosmology import cosmo as _c
+    cosmo = _c if cosmo is None else cosmo
+    hc2 = char_strain_sq_from_bin_edges_redz(edges, redz, cosmo=cosmo)
+    if realize in [None, False]:
+        F = hc2.shape[-1]
+        n = D.to_dev(number).reshape(-1, F).contiguous()
+        h = D.to_dev(hc2).reshape(-1, F).contiguous()
+        if not sum:
+            return np.sqrt(hc2 * number)
+        _, _, hb, _ = engine.realize(n, h, 1, 0, _lib.HD_SEL_NONE, counts=n.reshape(-1, F, 1).contiguous())
+        return np.sqrt(D.to_host(hb)[:, 0])
+    if utils.isinteger(realize):
+        if not sum:
+            raise NotImplementedError("per-bin realizations (sum=False) are not supported on the device path")
+        from .cyutils import sam_poisson_gwb
+        return np.sqrt(sam_poisson_gwb(number, hc2, realize, seed=seed))
+    raise ValueError("`realize` ({}) must be one of {{True, False, integer}}!".format(realize))

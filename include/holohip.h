@@ -1,0 +1,168 @@
+/*
+ * holohip.h -- C-ABI of libholohip.so, the MI355X (gfx950) implementation of
+ * holodeck's semi-analytic GW-background hot path.
+ *
+ * Conventions (every entry point):
+ *   - pointers are DEVICE pointers to contiguous row-major (C-order) arrays;
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream); every
+ *     call is asynchronous on that stream;
+ *   - the return value is a status: 0 = OK, <0 = error; hd_last_error() gives
+ *     a message for the calling thread;
+ *   - no entry point allocates memory visible to the caller; entry points that
+ *     need scratch take a caller-provided workspace whose size is reported by
+ *     the matching *_workspace_size() query;
+ *   - FP64 in and out, as the reference's typed memoryviews (double[:,...]).
+ *
+ * Each declaration cites the reference interface it replaces (holodeck v1.6,
+ * paths relative to the reference repository root).
+ */
+#ifndef HOLOHIP_H
+#define HOLOHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message for the last error on the calling thread ("" if none). */
+const char *hd_last_error(void);
+/* Library ABI version. */
+int hd_version(void);
+
+/* ------------------------------------------------------------------------
+ * Cosmology (flat LambdaCDM, no radiation).  Replaces the cosmopy/astropy
+ * `holo.cosmo` instance (holodeck/__init__.py:81-86): the interpolation grids
+ * `_grid_z/_grid_age/_grid_dcom` consumed by sam_cyutils.pyx:476,494 and the
+ * exact `age`, `tage_to_z`, `dtdz`, `comoving_distance` used on the path
+ * (sam.py:347, utils.py:1772-1808, gravwaves.py:713-714).
+ * ---------------------------------------------------------------------- */
+/* Comoving-distance Hermite table: dc_tab, dc_der each (n_tab+1) doubles,
+ * in units of c/H0, uniform in u = (1+z)^-1/2 on [0,1]. */
+int hd_cosmo_build_dcom_table(double om0, int n_tab, double *dc_tab, double *dc_der, void *stream);
+/* age [s] and comoving distance [cm] at the n redshifts z (table nodes). */
+int hd_cosmo_eval(double om0, double h, const double *dc_tab, const double *dc_der, int n_tab, const double *z,
+                  int64_t n, double *age, double *dcom, double *dtdz, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Static binary density  <- Semi_Analytic_Model.static_binary_density
+ * (holodeck/sams/sam.py:280-398) with GSMF_Schechter (components.py:110-172),
+ * GPF_Power_Law (520-583) / GMT_Power_Law (629-675) or GMR_Illustris
+ * (380-482), MMBulge_Standard + BF_Constant (host_relations.py:166-195,
+ * 624-771) and GMT redshift delay utils.redz_after (utils.py:1772-1808).
+ * Writes dens (M,Q,Z) [Mpc^-3] BEFORE scatter and before zeroing stalled
+ * bins, plus zprime (M,Q,Z) and gmt_time (M,Q,Z) [s].
+ * `par` is a HOST array of HD_DENS_NPAR doubles (layout:
+ * holodeck_amd/csrc/density_par.h); dc_tab/dc_der from
+ * hd_cosmo_build_dcom_table.
+ * ---------------------------------------------------------------------- */
+#define HD_DENS_NPAR 48
+int hd_static_density(const double *mtot, int M, const double *mrat, int Q, const double *redz, int Z,
+                      const double *par, const double *dc_tab, const double *dc_der, int n_tab,
+                      double *dens, double *zprime, double *gmt_time, void *stream);
+/* dens[zprime < 0] = 0  (sam.py:392-394) */
+int hd_zero_stalled(double *dens, const double *zprime, int64_t n, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Binary evolution  <- holodeck/sams/sam_cyutils.pyx
+ * ---------------------------------------------------------------------- */
+/* find_2pwl_hardening_norm (sam_cyutils.pyx:289-353): brentq per (mtot,mrat).
+ * norm (may be NULL) receives 10**norm_log10, as Fixed_Time_2PL_SAM stores
+ * it (hardening.py:1413-1416). */
+int hd_find_2pwl_norm(double target_time, const double *mtot, const double *mrat, int64_t n, double sepa_init,
+                      double rchar, double gamma_inner, double gamma_outer, int nsteps, double *norm_log10,
+                      double *norm, void *stream);
+/* hard_func_2pwl_gw (sam_cyutils.pyx:271-286), elementwise over n. */
+int hd_hard_func_2pwl_gw(const double *mtot, const double *mrat, const double *sepa, const double *norm, int64_t n,
+                         double rchar, double gamma_inner, double gamma_outer, double *dadt, void *stream);
+/* _dynamic_binary_number_at_fobs_gw (sam_cyutils.pyx:784-899). */
+int hd_dbn_gw(const double *fobs_orb, int F, const double *nden, const double *mtot, int M, const double *mrat, int Q,
+              int Z, const double *redz_prime, const double *grid_z, const double *grid_dcom, int n_interp,
+              double *redz_final, double *diff_num, void *stream);
+/* _dynamic_binary_number_at_fobs_2pwl (sam_cyutils.pyx:506-781).
+ * fobs_sorted != 0 promises fobs_orb is non-decreasing (enables a binary search). */
+int hd_dbn_2pwl(const double *fobs_orb, int F, int fobs_sorted, double sepa_init, int num_steps,
+                const double *hard_norm, double rchar, double gamma_inner, double gamma_outer, const double *nden,
+                const double *mtot, int M, const double *mrat, int Q, const double *redz, int Z,
+                const double *gmt_time, const double *grid_z, const double *grid_dcom, const double *grid_age,
+                int n_interp, double *redz_final, double *diff_num, void *stream);
+/* _integrate_differential_number_3dx1d (sam_cyutils.pyx:166-216):
+ * log10_mtot (M), mrat (Q), redz (Z), dln_freq (F), dnum (M,Q,Z,F) -> numb (M-1,Q-1,Z-1,F). */
+int hd_integrate_3dx1d(const double *log10_mtot, int M, const double *mrat, int Q, const double *redz, int Z,
+                       const double *dln_freq, int F, const double *dnum, double *numb, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Strain  <- gravwaves.char_strain_sq_from_bin_edges_redz (gravwaves.py:694-725)
+ * mt_cent (M-1), mr_cent (Q-1) bin centres; fc (F) orbital bin centres,
+ * fc_over_df (F); redz_final (M,Q,Z,F) -> h2fdf (M-1,Q-1,Z-1,F).
+ * gw_src_const = 8 G^(5/3) pi^(2/3) / sqrt(10) / c^4 (utils.py:40, astropy G, c).
+ * ---------------------------------------------------------------------- */
+int hd_h2fdf_redz(const double *mt_cent, int M, const double *mr_cent, int Q, int Z, const double *fc,
+                  const double *fc_over_df, int F, const double *redz_final, double gw_src_const, double om0, double h,
+                  const double *dc_tab, const double *dc_der, int n_tab, double *h2fdf, void *stream);
+
+/* Per-bin parameters for the loudest walk with parameters
+ * (single_sources.py:104-139): out (7, B, F) = [mt, mr, rz (bin centres),
+ * redz_final (midpointed; -1 where <= 0), dcom_final [cm] (+inf where
+ * redz_final <= 0), sepa_final [cm], angs_final [rad]]; fc (F) orbital centres. */
+int hd_bin_params_redz(const double *mt_cent, int M, const double *mr_cent, int Q, const double *rz_cent, int Z,
+                       const double *fc, int F, const double *redz_final, double om0, double h,
+                       const double *dc_tab, const double *dc_der, int n_tab, double *out, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Ordering  <- single_sources.ss_gws_redz's argsort(-h2fdf[..., 0])
+ * (single_sources.py:89-93), made stable: descending by key, ties by
+ * ascending bin index.  key (B, stride) column `col`; order (B) int64 out.
+ * ---------------------------------------------------------------------- */
+size_t hd_order_workspace_size(int64_t B);
+int hd_order_desc(const double *key, int64_t B, int64_t stride, int64_t col, int64_t *order, void *workspace,
+                  size_t ws_bytes, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Realizations (holodeck/cyutils.pyx).  number, h2fdf are (B,F) with B the
+ * flat C-order bin index over (M-1,Q-1,Z-1).
+ *
+ * Loud-source selection modes:
+ *   HD_SEL_SORTED : walk bins in `order` (loudest first); the first L source
+ *                   instances (with multiplicity) are the loud sources; the
+ *                   rest is background  <- _loudest_hc_from_sorted
+ *                   (cyutils.pyx:1266-1344); with HD_FLAG_SKIP_ZERO_H also
+ *                   _loudest_hc_and_par_from_sorted_redz (1615-1767).
+ *   HD_SEL_ARGMAX : per frequency, the L largest h2fdf instances among bins
+ *                   with n>0, ties to the lower bin index; L=1 is
+ *                   _ss_bg_hc(_and_par) (cyutils.pyx:935-1178), where the
+ *                   background is sum(n*h) - max.
+ *   HD_SEL_NONE   : no loud sources; hc2bg = sum(n*h)  <- _sam_poisson_gwb
+ *                   (cyutils.pyx:862-897).
+ * Outputs: hc2ss (F,R,L) and ssidx (F,R,L) (flat bin index, -1 if empty),
+ * hc2bg (F,R); bgpar (npar,F,R) strain-weighted background means of
+ * par (npar,B,F) when npar>0.
+ * ---------------------------------------------------------------------- */
+#define HD_SEL_SORTED 0
+#define HD_SEL_ARGMAX 1
+#define HD_SEL_NONE 2
+#define HD_FLAG_SKIP_ZERO_H 1
+
+/* Injected counts (B,F,R) (deterministic; the parity path). */
+size_t hd_realize_counts_workspace_size(int64_t B, int F, int R, int L, int npar);
+int hd_realize_counts(const double *counts, const double *h2fdf, const int64_t *order, int64_t B, int F, int R, int L,
+                      int mode, int flags, const double *par, int npar, double *hc2ss, int64_t *ssidx, double *hc2bg,
+                      double *bgpar, void *workspace, size_t ws_bytes, void *stream);
+
+/* On-GPU Poisson realizations (counter-based RNG keyed by seed and
+ * (bin, frequency, realization)); normal_threshold as the reference's 1e10. */
+size_t hd_realize_workspace_size(int64_t B, int F, int R, int L, int npar);
+int hd_realize(const double *number, const double *h2fdf, const int64_t *order, int64_t B, int F, int R, int L,
+               int mode, int flags, double normal_threshold, uint64_t seed, const double *par, int npar,
+               double *hc2ss, int64_t *ssidx, double *hc2bg, double *bgpar, void *workspace, size_t ws_bytes,
+               void *stream);
+
+/* Philox4x32-10 block function (host), for known-answer tests of the
+ * generator the realization kernels use. */
+void hd_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HOLOHIP_H */

@@ -1,0 +1,134 @@
+"""GPU parity: every kernel of the path against the FP64 CPU oracle on the same inputs.
+
+Tolerances: FP64 stage-A kernels 1e-9 relative (different libm pow/log ulps and
+one table-walk tie rule); realization with injected counts:
+h_c within 1e-5 relative (observed ~1e-13), loud-source
+bin indices bit-exact.
+"""
+import numpy as np
+import pytest
+
+from holodeck_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+SMALL = (17, 15, 13)
+
+
+def _close(a, b, rtol, atol=0.0, what=""):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    fin = np.isfinite(b)
+    assert np.array_equal(np.isfinite(a), fin), f"{what}: finite pattern differs"
+    err = np.abs(a[fin] - b[fin])
+    tol = atol + rtol * np.abs(b[fin])
+    bad = err > tol
+    assert not bad.any(), f"{what}: {bad.sum()} / {bad.size} outside rtol={rtol}; worst {np.max(err / (np.abs(b[fin]) + 1e-300))}"
+
+
+def _stageA(spec, gpu, oracle_lib):
+    from oracle.cfg import oracle_cfg
+    from holodeck_amd import device as D
+    sam = configs.make_sam(spec, scatter=0.0)
+    hard = configs.make_hard(spec, sam)
+    cfg = oracle_cfg(spec)
+    cfg['mmbulge']['scatter_dex'] = 0.0
+    o = oracle_lib.number_h2fdf(cfg)
+    fe = configs.fobs_gw_edges(spec)
+    number, h2fdf, rf = sam._number_h2fdf_device(fe, hard)
+    return sam, hard, o, D.to_host(number), D.to_host(h2fdf), D.to_host(rf)
+
+
+@pytest.mark.parametrize("which", ["config1", "config2"])
+def test_stageA_parity(which, gpu, oracle_lib):
+    print(f"[stageA {which}]", flush=True)
+    spec = configs.spec_config1(shape=SMALL, nfreq=7) if which == "config1" else configs.spec_config2(shape=SMALL,
+                                                                                                      nfreq=7)
+    sam, hard, o, number, h2fdf, rf = _stageA(spec, gpu, oracle_lib)
+    _close(sam.static_binary_density, o['dens'], 1e-12, what="density")
+    if 'norm_log10' in o:
+        from holodeck_amd import device as D
+        _close(D.to_host(hard._norm_log10_dev), o['norm_log10'], 1e-12, what="norm_log10")
+    _close(rf, o['redz_final'], 1e-9, what="redz_final")
+    F = number.shape[-1]
+    _close(number, o['number'].reshape(-1, F), 1e-9, atol=0.0, what="number")
+    _close(h2fdf, o['h2fdf'].reshape(-1, F), 1e-9, what="h2fdf")
+
+
+def _rand_counts(number, R, seed):
+    rng = np.random.default_rng(seed)
+    lam = np.minimum(number, 1e6)
+    return rng.poisson(lam[..., None] * np.ones(R)).astype(float)
+
+
+def test_realize_counts_sorted(gpu, oracle_lib):
+    print("[realize counts sorted]", flush=True)
+    from holodeck_amd import engine, _lib
+    from holodeck_amd import device as D
+    rng = np.random.default_rng(1)
+    B, F, R, L = 3000, 5, 7, 4
+    h2 = 10.0 ** rng.uniform(-35, -25, size=(B, F))
+    h2[rng.random((B, F)) < 0.05] = 0.0
+    number = 10.0 ** rng.uniform(-4, 1, size=(B, F))
+    counts = _rand_counts(number, R, 2)
+    order = oracle_lib.order_desc(h2[:, 0])
+    for skip in (False, True):
+        ss_o, idx_o, bg_o, _ = oracle_lib.loudest_counts(counts, h2, order, L, skip_zero_h=skip)
+        ss, idx, bg, _ = engine.realize(D.to_dev(number), D.to_dev(h2), R, L, _lib.HD_SEL_SORTED,
+                                        order=D.to_dev(order), counts=D.to_dev(counts),
+                                        flags=_lib.HD_FLAG_SKIP_ZERO_H if skip else 0)
+        assert np.array_equal(D.to_host(idx), idx_o)
+        _close(D.to_host(ss), ss_o, 0.0, what="hc2ss")
+        _close(D.to_host(bg), bg_o, 1e-12, what="hc2bg")
+
+
+def test_realize_counts_argmax_and_none(gpu, oracle_lib):
+    print("[realize counts argmax/none]", flush=True)
+    from holodeck_amd import engine, _lib
+    from holodeck_amd import device as D
+    rng = np.random.default_rng(3)
+    B, F, R = 2500, 4, 6
+    h2 = 10.0 ** rng.uniform(-35, -25, size=(B, F))
+    number = 10.0 ** rng.uniform(-4, 0.5, size=(B, F))
+    counts = _rand_counts(number, R, 4)
+    par = rng.uniform(1, 2, size=(3, B))
+    ss_o, bg_o, idx_o, bgp_o = oracle_lib.ss_bg_counts(counts, h2, par=par)
+    pdev = D.to_dev(np.repeat(par[:, :, None], F, axis=2))
+    ss, idx, bg, bgp = engine.realize(D.to_dev(number), D.to_dev(h2), R, 1, _lib.HD_SEL_ARGMAX,
+                                      counts=D.to_dev(counts), par=pdev)
+    assert np.array_equal(D.to_host(idx)[..., 0], idx_o)
+    _close(D.to_host(ss)[..., 0], ss_o, 0.0, what="ss argmax")
+    _close(D.to_host(bg), bg_o, 1e-10, what="bg argmax")
+    _close(D.to_host(bgp), bgp_o, 1e-10, what="bgpar argmax")
+    gwb_o = oracle_lib.poisson_gwb_counts(counts, h2)
+    _, _, bgn, _ = engine.realize(D.to_dev(number), D.to_dev(h2), R, 0, _lib.HD_SEL_NONE, counts=D.to_dev(counts))
+    _close(D.to_host(bgn), gwb_o, 1e-12, what="poisson gwb")
+
+
+def test_order_matches_stable_argsort(gpu, oracle_lib):
+    print("[order]", flush=True)
+    from holodeck_amd import engine
+    from holodeck_amd import device as D
+    rng = np.random.default_rng(5)
+    for B in (1, 7, 4096, 4097, 100003):
+        key = 10.0 ** rng.uniform(-40, -20, size=(B, 3))
+        key[rng.random((B, 3)) < 0.2] = 0.0          # many ties
+        key[rng.random((B, 3)) < 0.05] = 1e-30        # more ties
+        o = engine.order_desc(D.to_dev(key), col=1)
+        assert np.array_equal(D.to_host(o), oracle_lib.order_desc(key[:, 1])), B
+
+
+def test_pipeline_counts_end_to_end(gpu, oracle_lib):
+    """gwb with injected counts vs the oracle pipeline: h_c 1e-5 relative (observed ~1e-12), indices exact."""
+    print("[pipeline injected counts]", flush=True)
+    spec = configs.spec_config2(shape=SMALL, nfreq=6)
+    sam, hard, o, number, h2fdf, rf = _stageA(spec, gpu, oracle_lib)
+    F = number.shape[-1]
+    R, L = 5, 3
+    counts = _rand_counts(o['number'].reshape(-1, F), R, 9)
+    order = oracle_lib.order_desc(o['h2fdf'].reshape(-1, F)[:, 0])
+    ss_o, idx_o, bg_o, _ = oracle_lib.loudest_counts(counts, o['h2fdf'].reshape(-1, F), order, L)
+    hc_ss, hc_bg = sam.gwb(configs.fobs_gw_edges(spec), hard=hard, realize=R, loudest=L, counts=counts)
+    _close(hc_bg, np.sqrt(bg_o), 1e-5, what="hc_bg")
+    _close(hc_ss, np.sqrt(ss_o), 1e-5, what="hc_ss")

@@ -1,0 +1,93 @@
+"""CPU-only checks of the C-ABI library and the host logic (no GPU compute)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "holohip.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(hd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from holodeck_amd import _lib
+    lib = _lib.load()
+    names = _declared()
+    assert len(names) >= 15
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines() if " T " in line)
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert getattr(lib, n) is not None
+    # every declared entry point has a ctypes signature
+    assert not [n for n in names if n not in _lib.SIGNATURES]
+
+
+def test_philox_known_answers():
+    """Random123 philox4x32_10 known-answer vectors (kat_vectors)."""
+    from holodeck_amd import _lib
+    assert _lib.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert _lib.philox4x32_10([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6,
+                                                                       0x6d5451fd]
+    assert _lib.philox4x32_10([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_errors_are_reported_without_gpu():
+    """Argument validation happens before any device work and reports through hd_last_error."""
+    from holodeck_amd import _lib
+    with pytest.raises(_lib.HoloHipError, match="bad arguments"):
+        _lib.call("hd_find_2pwl_norm", 1.0, None, None, 5, 1.0, 1.0, -1.0, 2.5, 300, None, None, None)
+    with pytest.raises(_lib.HoloHipError, match="L=99"):
+        _lib.call("hd_realize_counts", 1, 1, None, 10, 2, 3, 99, 0, 0, None, 0, 1, 1, 1, None, None, 0, None)
+
+
+def test_pta_freqs_and_midpoints():
+    from holodeck_amd import utils
+    from holodeck_amd.constants import YR
+    cents, edges = utils.pta_freqs(16.03 * YR, 40)
+    np.testing.assert_allclose(cents, np.arange(1, 41) / (16.03 * YR))
+    np.testing.assert_allclose(utils.midpoints(edges), cents, rtol=1e-14)
+    assert utils.isinteger(5) and not utils.isinteger(False.__class__) and not utils.isinteger(5.0)
+
+
+def test_component_conversions_match_reference_definitions():
+    from holodeck_amd import sams, host_relations
+    from holodeck_amd.constants import MSOL, GYR
+    gpf = sams.GPF_Power_Law(frac_norm_allq=0.025)
+    assert np.isclose(gpf._frac_norm, 0.025 / 0.75)            # components.py:531-539
+    gmt = sams.GMT_Power_Law(cosmo=type("C", (), {"h": 0.5})())
+    assert np.isclose(gmt._mref, 1e11 * MSOL * 0.8)             # components.py:646
+    gmr = sams.GMR_Illustris()
+    assert np.isclose(gmr._norm0, 10 ** -2.2287 / GYR)
+    mm = host_relations.MMBulge_KH2013()
+    assert np.isclose(mm._mamp, MSOL * 10 ** 8.69) and mm._mplaw == 1.17 and mm._scatter_dex == 0.28
+    assert mm._bulge_frac.bulge_frac() == 0.615
+    g = sams.GSMF_Schechter(mchar0_log10=10.84, mcharz=0.11)
+    assert np.isclose(g._mchar0, MSOL * 10 ** 10.84) and g._mcharz == 0.11
+
+
+def test_density_param_layout_matches_kernel_enum():
+    """The packed parameter vector follows the index enum of density_par.h."""
+    from holodeck_amd import sams, host_relations, cosmology
+    from holodeck_amd.sams.components import density_params
+    hdr = open(os.path.join(ROOT, "holodeck_amd", "csrc", "density_par.h")).read()
+    names = re.findall(r"^\s+(DP_[A-Z0-9_]+)", hdr, flags=re.M)
+    pos = {n: i for i, n in enumerate(names)}
+    c = cosmology.Cosmology(h=0.7, Om0=0.3)
+    p = density_params(sams.GSMF_Schechter(), sams.GPF_Power_Law(), sams.GMT_Power_Law(cosmo=c), None,
+                       host_relations.MMBulge_KH2013(), c)
+    assert p[pos['DP_GSMF_PHI0']] == -2.77 and p[pos['DP_USE_GMR']] == 0 and p[pos['DP_HAS_GMT']] == 1
+    assert p[pos['DP_MMB_PLAW']] == 1.17 and p[pos['DP_COSMO_OM0']] == 0.3 and p[pos['DP_COSMO_H']] == 0.7
+    assert p[pos['DP_GPF_MAXFRAC']] == 1.0 and p[pos['DP_MMB_BFRAC']] == 0.615
+    p2 = density_params(sams.GSMF_Schechter(), None, None, sams.GMR_Illustris(), host_relations.MMBulge_KH2013(), c)
+    assert p2[pos['DP_USE_GMR']] == 1 and p2[pos['DP_HAS_GMT']] == 0
+    assert p2[pos['DP_GMR_MREF']] == sams.GMR_Illustris()._mref
