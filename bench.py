@@ -1,0 +1,155 @@
+#!/usr/bin/env python
+"""Benchmark: GWB spectra/s of the plain GPU SAM on one MI355X.
+
+A step is one complete spectrum of the full-size workload from its model
+parameters: static density with M-Mbulge scatter, Fixed_Time_2PL_SAM
+hardening normalisations (Brent), 2PL dynamic binary number, bin integration,
+strain, frequency-0 order, 500 Poisson realizations with the 10 loudest
+sources per frequency split from the background.  Inputs are parameters and
+1-D grids; every per-cell value is computed inside the timed region.
+
+Prints one JSON line (rank 0).  --gpus N>1 is rejected: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GWB spectra/sec on 1 MI355X: plain GPU SAM vs MLP surrogate at ≤0.05 dex err"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(spec, nz_sample=11, r_sample=20):
+    """Oracle (FP64 C restatement, 1 thread) on a bounded sample of the workload, scaled to spectra/s.
+
+    Sample: the same model on an Mx Q x nz_sample grid (stage A: density, scatter, Brent norms,
+    2PL walk, integration, strain, order) and r_sample realizations of that grid.  Stage-A time
+    is scaled by the number of redshift bins (the Brent norms do not depend on z), the realization
+    time by cell-realizations B*F*R.
+    """
+    import oracle
+    from oracle.cfg import oracle_cfg
+    from oracle.scatter import add_scatter_to_masses
+    full = oracle_cfg(spec)
+    M, Q, Z = full['shape']
+    cfg = dict(full)
+    cfg['shape'] = (M, Q, nz_sample)
+    c = cfg['cosmo']
+    t0 = time.perf_counter()
+    cos = oracle.OracleCosmo(h=c['h'], Om0=c['Om0'])
+    mtot, mrat, redz = oracle.edges_from_cfg(cfg)
+    dens, zp, gmt = oracle.static_density(mtot, mrat, redz, cfg, cos)
+    dens = add_scatter_to_masses(mtot, mrat, dens, cfg['mmbulge']['scatter_dex'])
+    dens[zp < 0] = 0.0
+    h = cfg['hard']
+    mt2, mr2 = np.meshgrid(mtot, mrat, indexing='ij')
+    tn0 = time.perf_counter()
+    nl = oracle.find_2pwl_norm(h['time'], mt2.ravel(), mr2.ravel(), h['sepa_init'], h['rchar'], h['gamma_inner'],
+                               h['gamma_outer'], h['num_steps']).reshape(mt2.shape)
+    t_norm = time.perf_counter() - tn0
+    fe = cfg['fobs_gw_edges']
+    fo = oracle.midpoints(fe) / 2.0
+    rf, dn = oracle.dbn_2pwl(fo, h['sepa_init'], h['num_steps'], 10.0 ** nl, h['rchar'], h['gamma_inner'],
+                             h['gamma_outer'], dens, mtot, mrat, redz, gmt, cos)
+    edges = [mtot, mrat, redz, fe / 2.0]
+    num = oracle.integrate_3dx1d(edges, dn)
+    h2 = oracle.h2fdf_redz(edges, rf, cos)
+    F = num.shape[-1]
+    n2, hh = num.reshape(-1, F), h2.reshape(-1, F)
+    order = oracle.order_desc(hh[:, 0])
+    t_a = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    oracle.loudest_rng(n2, hh, order, r_sample, 10, 12345)
+    t_b = time.perf_counter() - t1
+    zscale = (Z - 1) / (nz_sample - 1)
+    t_a_full = t_norm + (t_a - t_norm) * zscale
+    B_full = (M - 1) * (Q - 1) * (Z - 1)
+    R_full = 500
+    cr_rate = n2.size * r_sample / t_b
+    t_b_full = B_full * F * R_full / cr_rate
+    return dict(value=1.0 / (t_a_full + t_b_full), unit="spectra/s", cores=1, kind="port",
+                sample=(f"oracle (FP64 C/numpy, 1 thread) on the same model with {nz_sample} redshift edges and "
+                        f"{r_sample} realizations: stage A {t_a:.1f}s (scaled x{zscale:.1f} in z, Brent norms "
+                        f"{t_norm:.1f}s unscaled), realizations {t_b:.1f}s at {cr_rate:.3g} cell*real/s scaled to "
+                        f"B*F*R={B_full * F * R_full:.3g}"),
+                stage_a_s=t_a_full, realize_s=t_b_full, measured_s=t_a + t_b)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nreals", type=int, default=500)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(parameter-space samples shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from holodeck_amd import configs, engine
+    from holodeck_amd.cosmology import Cosmology
+
+    spec = configs.spec_config2()
+    M, Q, Z = spec['shape']
+    F, R, L = spec['nfreq'], args.nreals, 10
+    B = (M - 1) * (Q - 1) * (Z - 1)
+    fe = configs.fobs_gw_edges(spec)
+    cos = Cosmology(h=spec['cosmo']['h'], Om0=spec['cosmo']['Om0'])
+    cos.device_tables()
+
+    def step(seed):
+        sam = configs.make_sam(spec, cosmo=cos)
+        hard = configs.make_hard(spec, sam)
+        return sam.gwb_device(fe, hard, R, L, seed=seed)
+
+    for w in range(args.warmup):
+        out = step(1000 + w)
+    torch.cuda.synchronize()
+    engine.EVENTS = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        out = step(s)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stages = engine.stage_times_ms()
+    engine.EVENTS = None
+    hc2bg = out[2]
+    assert torch.isfinite(hc2bg).all() and bool((hc2bg > 0).all())
+
+    ms = dt / args.steps * 1e3
+    value = args.steps / dt
+    t_real = stages.get("realize", float("nan")) * 1e-3
+    alg_bytes = 16.0 * B * F          # number + h2fdf (FP64) read once for all realizations
+    achieved = alg_bytes / t_real / 1e9
+    line = {
+        "metric": METRIC, "value": value, "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": ("full-size plain SAM: 91x81x101 grid (mtot 1e4-1e12 Msol, q 1e-3-1, z 1e-3-10), "
+                                "40 frequencies i/T (T=16.03 yr), flat LCDM H0=67.7 Om=0.31, GSMF Schechter + "
+                                "GPF/GMT power laws, KH2013 M-Mbulge with 0.28 dex scatter, Fixed_Time_2PL_SAM "
+                                "(1 Gyr, -1, +2.5, 100 pc, 1e4 pc), 500 realizations, 10 loudest"),
+                   "grid": [M, Q, Z], "nfreq": F, "nreals": R, "nloudest": L},
+        "cell_realizations_per_s": B * F * R * value,
+        "stages_ms": stages,
+        "roofline": {"bound": "hbm", "kernel": "realize", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(spec)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -31,9 +31,9 @@ def fobs_gw_edges(spec):
     return utils.pta_freqs(spec['dur'], spec['nfreq'])[1]
 
 
-def make_sam(spec, scatter=None):
+def make_sam(spec, scatter=None, cosmo=None):
     import holodeck_amd as holo
-    cos = holo.Cosmology(h=spec['cosmo']['h'], Om0=spec['cosmo']['Om0'])
+    cos = cosmo if cosmo is not None else holo.Cosmology(h=spec["cosmo"]["h"], Om0=spec["cosmo"]["Om0"])
     gsmf = holo.sams.GSMF_Schechter(**spec['gsmf'])
     kw = {}
     if spec.get('gmr') is not None:

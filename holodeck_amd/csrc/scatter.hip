@@ -1,0 +1,311 @@
+// scatter.hip -- M-Mbulge scatter of the binary density (FP64), all redshift slices at once.
+//
+// <- add_scatter_to_masses (holodeck/sams/sam.py:1291-1394).  Per slice:
+//   1. global gradient estimation of the Clough-Tocher interpolant at the
+//      (log10 m1, log10 m2) nodes: the Gauss-Seidel minimisation of the edge
+//      curvature (G. M. Nielson, Math. Comp. 40 (1983) 253; R. J. Renka and
+//      A. K. Cline, Rocky Mountain J. Math. 14 (1984) 223), tol 1e-6, at most
+//      400 sweeps, as scipy's CloughTocher2DInterpolator defaults;
+//   2. Clough-Tocher evaluation on the regular (G x G) log-mass grid (cubic
+//      Bernstein-Bezier patches on the three micro-triangles, affine-invariant
+//      cross-boundary derivatives; P. Alfeld, CAGD 1 (1984) 169; G. Farin,
+//      CAGD 3 (1986) 83), NaN/negative -> nearest node value;
+//   3. Y = P D with P = (W W)^T, the reference's two einsum passes (both act on
+//      the m1 axis), as one FP64 GEMM over all slices;
+//   4. bilinear interpolation back to the nodes.
+//
+// The Gauss-Seidel sweep is level-scheduled: a vertex's level is one more than
+// the highest level of its lower-indexed neighbours, so updating whole levels
+// in order reads exactly the values the sequential sweep reads (new values for
+// lower indices, previous-sweep values for higher ones).  One wavefront runs
+// one slice with the solution vector in LDS.
+#include "common.hpp"
+#include "../../include/holohip.h"
+
+#pragma clang fp contract(off)
+
+namespace hd {
+
+template <bool LDS_Y>
+__global__ void __launch_bounds__(64) k_ct_gradients(const double* __restrict__ dens, int N, int Z,
+                                                     const double* __restrict__ pts, const int32_t* __restrict__ nptr,
+                                                     const int32_t* __restrict__ nidx, const int32_t* __restrict__ lptr,
+                                                     const int32_t* __restrict__ lidx, int n_levels, double tol,
+                                                     int maxiter, double* __restrict__ grad_g,
+                                                     int32_t* __restrict__ iters_out) {
+    extern __shared__ double s_y[];
+    const int z = blockIdx.x;
+    const int lane = threadIdx.x;
+    double* y = LDS_Y ? s_y : (grad_g + (int64_t)z * N * 2);
+    for (int i = lane; i < 2 * N; i += 64) y[i] = 0.0;
+    __syncthreads();
+    int iters = 0;
+    for (int it = 0; it < maxiter; it++) {
+        double err = 0.0;
+        for (int lev = 0; lev < n_levels; lev++) {
+            const int l0 = lptr[lev], l1 = lptr[lev + 1];
+            for (int t = l0 + lane; t < l1; t += 64) {
+                const int p = lidx[t];
+                const double px = pts[2 * p], py = pts[2 * p + 1];
+                const double f1 = dens[(int64_t)p * Z + z];
+                double Q0 = 0, Q1 = 0, Q3 = 0, s0 = 0, s1 = 0;
+                for (int jj = nptr[p]; jj < nptr[p + 1]; jj++) {
+                    const int q = nidx[jj];
+                    const double ex = pts[2 * q] - px;
+                    const double ey = pts[2 * q + 1] - py;
+                    const double L = sqrt(ex * ex + ey * ey);
+                    const double L3 = L * L * L;
+                    const double f2 = dens[(int64_t)q * Z + z];
+                    const double df2 = -ex * y[2 * q] - ey * y[2 * q + 1];
+                    Q0 += 4 * ex * ex / L3;
+                    Q1 += 4 * ex * ey / L3;
+                    Q3 += 4 * ey * ey / L3;
+                    s0 += (6 * (f1 - f2) - 2 * df2) * ex / L3;
+                    s1 += (6 * (f1 - f2) - 2 * df2) * ey / L3;
+                }
+                const double Q2 = Q1;
+                const double det = Q0 * Q3 - Q1 * Q2;
+                const double r0 = (Q3 * s0 - Q1 * s1) / det;
+                const double r1 = (-Q2 * s0 + Q0 * s1) / det;
+                double a0 = fabs(y[2 * p] + r0), a1 = fabs(y[2 * p + 1] + r1);
+                double change = (a0 > a1) ? a0 : a1;
+                y[2 * p] = -r0;
+                y[2 * p + 1] = -r1;
+                double m0 = fabs(r0), m1 = fabs(r1);
+                double mr = (m0 > m1) ? m0 : m1;
+                change /= (1.0 > mr) ? 1.0 : mr;
+                err = (err > change) ? err : change;
+            }
+            if (LDS_Y) __syncthreads();
+            else __threadfence_block();
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            double o = __shfl_xor(err, off, 64);
+            err = (err > o) ? err : o;
+        }
+        if (err < tol) {
+            iters = it + 1;
+            break;
+        }
+    }
+    if (LDS_Y)
+        for (int i = lane; i < 2 * N; i += 64) grad_g[(int64_t)z * N * 2 + i] = y[i];
+    if (lane == 0) iters_out[z] = iters;
+}
+
+// Clough-Tocher value at barycentric b in triangle s (vertex order of `simp`).
+__device__ double ct_value(const double* __restrict__ pts, const int32_t* __restrict__ simp, const double* g3,
+                           const double* b, const double* f, const double* df) {
+    const int v0 = simp[0], v1 = simp[1], v2 = simp[2];
+    const double e12x = pts[2 * v1] - pts[2 * v0], e12y = pts[2 * v1 + 1] - pts[2 * v0 + 1];
+    const double e23x = pts[2 * v2] - pts[2 * v1], e23y = pts[2 * v2 + 1] - pts[2 * v1 + 1];
+    const double e31x = pts[2 * v0] - pts[2 * v2], e31y = pts[2 * v0 + 1] - pts[2 * v2 + 1];
+    const double f1 = f[0], f2 = f[1], f3 = f[2];
+    const double df12 = +(df[0] * e12x + df[1] * e12y);
+    const double df21 = -(df[2] * e12x + df[3] * e12y);
+    const double df23 = +(df[2] * e23x + df[3] * e23y);
+    const double df32 = -(df[4] * e23x + df[5] * e23y);
+    const double df31 = +(df[4] * e31x + df[5] * e31y);
+    const double df13 = -(df[0] * e31x + df[1] * e31y);
+    const double c3000 = f1;
+    const double c2100 = (df12 + 3 * c3000) / 3;
+    const double c2010 = (df13 + 3 * c3000) / 3;
+    const double c0300 = f2;
+    const double c1200 = (df21 + 3 * c0300) / 3;
+    const double c0210 = (df23 + 3 * c0300) / 3;
+    const double c0030 = f3;
+    const double c1020 = (df31 + 3 * c0030) / 3;
+    const double c0120 = (df32 + 3 * c0030) / 3;
+    const double c2001 = (c2100 + c2010 + c3000) / 3;
+    const double c0201 = (c1200 + c0300 + c0210) / 3;
+    const double c0021 = (c1020 + c0120 + c0030) / 3;
+    const double c0111 = (g3[0] * (-c0300 + 3 * c0210 - 3 * c0120 + c0030) + (-c0300 + 2 * c0210 - c0120 + c0021 + c0201)) / 2;
+    const double c1011 = (g3[1] * (-c0030 + 3 * c1020 - 3 * c2010 + c3000) + (-c0030 + 2 * c1020 - c2010 + c2001 + c0021)) / 2;
+    const double c1101 = (g3[2] * (-c3000 + 3 * c2100 - 3 * c1200 + c0300) + (-c3000 + 2 * c2100 - c1200 + c2001 + c0201)) / 2;
+    const double c1002 = (c1101 + c1011 + c2001) / 3;
+    const double c0102 = (c1101 + c0111 + c0201) / 3;
+    const double c0012 = (c1011 + c0111 + c0021) / 3;
+    const double c0003 = (c1002 + c0102 + c0012) / 3;
+    double minval = b[0];
+    for (int k = 0; k < 3; k++)
+        if (b[k] < minval) minval = b[k];
+    const double b1 = b[0] - minval, b2 = b[1] - minval, b3 = b[2] - minval, b4 = 3 * minval;
+    return (b1 * b1 * b1 * c3000 + 3 * b1 * b1 * b2 * c2100 + 3 * b1 * b1 * b3 * c2010 + 3 * b1 * b1 * b4 * c2001 +
+            3 * b1 * b2 * b2 * c1200 + 6 * b1 * b2 * b4 * c1101 + 3 * b1 * b3 * b3 * c1020 + 6 * b1 * b3 * b4 * c1011 +
+            3 * b1 * b4 * b4 * c1002 + b2 * b2 * b2 * c0300 + 3 * b2 * b2 * b3 * c0210 + 3 * b2 * b2 * b4 * c0201 +
+            3 * b2 * b3 * b3 * c0120 + 6 * b2 * b3 * b4 * c0111 + 3 * b2 * b4 * b4 * c0102 + b3 * b3 * b3 * c0030 +
+            3 * b3 * b3 * b4 * c0021 + 3 * b3 * b4 * b4 * c0012 + b4 * b4 * b4 * c0003);
+}
+
+// One thread per (regular-grid point q, slice z); writes D[i][z][j] for q = i*G + j.
+__global__ void __launch_bounds__(256) k_ct_eval(const double* __restrict__ dens, int N, int Z, int G,
+                                                 const double* __restrict__ pts, const int32_t* __restrict__ simplices,
+                                                 const double* __restrict__ sg, const int32_t* __restrict__ qs,
+                                                 const double* __restrict__ qb, const int32_t* __restrict__ qn,
+                                                 const double* __restrict__ grad, double* __restrict__ Dm,
+                                                 int32_t* __restrict__ bad) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (q >= (int64_t)G * G) return;
+    const int s = qs[q];
+    double w = __builtin_nan("");
+    if (s >= 0) {
+        const int32_t* sp = simplices + 3 * s;
+        double f[3], df[6], b[3], g3[3];
+        for (int k = 0; k < 3; k++) {
+            const int v = sp[k];
+            f[k] = dens[(int64_t)v * Z + z];
+            df[2 * k] = grad[((int64_t)z * N + v) * 2];
+            df[2 * k + 1] = grad[((int64_t)z * N + v) * 2 + 1];
+            b[k] = qb[3 * q + k];
+            g3[k] = sg[3 * s + k];
+        }
+        w = ct_value(pts, sp, g3, b, f, df);
+    }
+    if (isnan(w) || w < 0.0) {
+        w = dens[(int64_t)qn[q] * Z + z];
+        if (isnan(w) || w < 0.0) atomicAdd(bad, 1);
+    }
+    const int64_t i = q / G, j = q % G;
+    Dm[(i * Z + z) * G + j] = w;
+}
+
+// C (m x n) = A (m x k) * B (k x n), row-major FP64; 64x64 tiles, 256 threads x (4x4) outputs.
+constexpr int GT = 64, GK = 16;
+__global__ void __launch_bounds__(256) k_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
+                                                  double* __restrict__ C, int m, int n, int k) {
+    __shared__ double sA[GK][GT + 1];
+    __shared__ double sB[GK][GT + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
+    double acc[4][4] = {{0}};
+    for (int k0 = 0; k0 < k; k0 += GK) {
+        for (int e = threadIdx.x; e < GT * GK; e += 256) {
+            int r = e / GK, kk = e % GK;
+            int gr = row0 + r, gk = k0 + kk;
+            sA[kk][r] = (gr < m && gk < k) ? A[(int64_t)gr * k + gk] : 0.0;
+            int kk2 = e / GT, c = e % GT;
+            int gk2 = k0 + kk2, gc = col0 + c;
+            sB[kk2][c] = (gk2 < k && gc < n) ? B[(int64_t)gk2 * n + gc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; kk++) {
+            double a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = sA[kk][ty * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bb[j] = sB[kk][tx * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], bb[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int r = row0 + ty * 4 + i;
+        if (r >= m) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int c = col0 + tx * 4 + j;
+            if (c < n) C[(int64_t)r * n + c] = acc[i][j];
+        }
+    }
+}
+
+// dens[p][z] = sum_c w[p][c] * Y[i_c][z][j_c]
+__global__ void __launch_bounds__(256) k_back_interp(const double* __restrict__ Y, int N, int Z, int G,
+                                                     const int32_t* __restrict__ bidx, const double* __restrict__ bw,
+                                                     double* __restrict__ dens) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * Z) return;
+    const int64_t p = t / Z;
+    const int z = (int)(t % Z);
+    double v = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int64_t q = bidx[4 * p + c];
+        const int64_t i = q / G, j = q % G;
+        v += bw[4 * p + c] * Y[(i * Z + z) * G + j];
+    }
+    dens[p * Z + z] = v;
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" int hd_gemm_f64(const double* A, const double* B, double* C, int m, int n, int k, void* stream) {
+    if (m < 1 || n < 1 || k < 1 || !A || !B || !C) return set_error(HD_ERR_ARG, "hd_gemm_f64: bad arguments");
+    dim3 grid((unsigned)cdiv(n, GT), (unsigned)cdiv(m, GT));
+    hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, (hipStream_t)stream, A, B, C, m, n, k);
+    return check_launch("k_gemm_f64");
+}
+
+extern "C" size_t hd_scatter_workspace_size(int M, int Q, int Z, int G) {
+    size_t N = (size_t)M * Q;
+    return align256(N * Z * 2 * 8) + 2 * align256((size_t)G * G * Z * 8) + 256 + 256;
+}
+
+extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double* pts, const int32_t* nbr_ptr,
+                          const int32_t* nbr_idx, const int32_t* levels_ptr, const int32_t* levels_idx, int n_levels,
+                          const int32_t* simplices, const double* simplex_g, const int32_t* q_simplex,
+                          const double* q_bary, const int32_t* q_nearest, const double* P, const int32_t* back_idx,
+                          const double* back_w, double tol, int maxiter, void* workspace, size_t ws_bytes,
+                          int32_t* iters_out, void* stream) {
+    if (M < 2 || Q < 2 || Z < 1 || G < 2 || n_levels < 1 || !dens || !pts || !nbr_ptr || !nbr_idx || !levels_ptr ||
+        !levels_idx || !simplices || !simplex_g || !q_simplex || !q_bary || !q_nearest || !P || !back_idx || !back_w ||
+        !iters_out)
+        return set_error(HD_ERR_ARG, "hd_scatter: bad arguments");
+    if ((int64_t)G * G > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_scatter: G too large");
+    size_t need = hd_scatter_workspace_size(M, Q, Z, G);
+    if (!workspace || ws_bytes < need) return set_error(HD_ERR_SIZE, "hd_scatter: workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    const int N = M * Q;
+    char* p = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+    double* grad = (double*)p;  p += align256((size_t)N * Z * 2 * 8);
+    double* Dm = (double*)p;    p += align256((size_t)G * G * Z * 8);
+    double* Y = (double*)p;     p += align256((size_t)G * G * Z * 8);
+    int32_t* bad = iters_out + Z;   // iters_out has Z + 1 entries
+
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, bad, 1);
+    // solution vector in LDS when it fits
+    int dev = 0;
+    hipGetDevice(&dev);
+    int max_lds = 0;
+    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    size_t y_bytes = (size_t)N * 2 * sizeof(double);
+    if (y_bytes <= (size_t)max_lds) {
+        if (y_bytes > 65536)
+            hipFuncSetAttribute((const void*)k_ct_gradients<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)y_bytes);
+        hipLaunchKernelGGL(k_ct_gradients<true>, dim3(Z), dim3(64), y_bytes, st, dens, N, Z, pts, nbr_ptr, nbr_idx,
+                           levels_ptr, levels_idx, n_levels, tol, maxiter, grad, iters_out);
+    } else {
+        hipLaunchKernelGGL(k_ct_gradients<false>, dim3(Z), dim3(64), 0, st, dens, N, Z, pts, nbr_ptr, nbr_idx,
+                           levels_ptr, levels_idx, n_levels, tol, maxiter, grad, iters_out);
+    }
+    int rc = check_launch("k_ct_gradients");
+    if (rc) return rc;
+    dim3 g1((unsigned)cdiv((int64_t)G * G, 256), Z);
+    hipLaunchKernelGGL(k_ct_eval, g1, dim3(256), 0, st, dens, N, Z, G, pts, simplices, simplex_g, q_simplex, q_bary,
+                       q_nearest, grad, Dm, bad);
+    rc = check_launch("k_ct_eval");
+    if (rc) return rc;
+    // Y (G x Z*G) = P (G x G) * D (G x Z*G)
+    rc = hd_gemm_f64(P, Dm, Y, G, Z * G, G, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_back_interp, dim3((unsigned)cdiv((int64_t)N * Z, 256)), dim3(256), 0, st, Y, N, Z, G, back_idx,
+                       back_w, dens);
+    return check_launch("k_back_interp");
+}

@@ -15,6 +15,33 @@ from .constants import NWTG, SPLC
 GW_SRC_CONST = 8 * np.power(NWTG, 5 / 3) * np.power(np.pi, 2 / 3) / np.sqrt(10) / np.power(SPLC, 4)
 NORMAL_THRESHOLD = 1e10
 
+#: when a dict, each stage records (start, end) HIP events on the current stream under its name
+EVENTS = None
+
+
+class _stage:
+    def __init__(self, name):
+        self.name = name
+
+    def __enter__(self):
+        if EVENTS is not None:
+            self.e0 = torch.cuda.Event(enable_timing=True)
+            self.e0.record()
+        return self
+
+    def __exit__(self, *exc):
+        if EVENTS is not None:
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record()
+            EVENTS.setdefault(self.name, []).append((self.e0, e1))
+        return False
+
+
+def stage_times_ms():
+    """Mean elapsed ms per recorded stage (synchronizes)."""
+    torch.cuda.synchronize()
+    return {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in (EVENTS or {}).items()}
+
 
 def find_2pwl_norm(target_time, mtot, mrat, sepa_init, rchar, gamma_inner, gamma_outer, nsteps):
     """(log10 norm, norm) per element (sam_cyutils.pyx:289-353, hardening.py:1413-1416)."""
@@ -22,9 +49,10 @@ def find_2pwl_norm(target_time, mtot, mrat, sepa_init, rchar, gamma_inner, gamma
     mr = D.to_dev(mrat)
     out = D.empty(mt.numel())
     norm = D.empty(mt.numel())
-    _lib.call("hd_find_2pwl_norm", float(target_time), D.ptr(mt), D.ptr(mr), mt.numel(), float(sepa_init),
-              float(rchar), float(gamma_inner), float(gamma_outer), int(nsteps), D.ptr(out), D.ptr(norm),
-              D.stream_ptr())
+    with _stage("norm_2pwl"):
+        _lib.call("hd_find_2pwl_norm", float(target_time), D.ptr(mt), D.ptr(mr), mt.numel(), float(sepa_init),
+                  float(rchar), float(gamma_inner), float(gamma_outer), int(nsteps), D.ptr(out), D.ptr(norm),
+                  D.stream_ptr())
     return out, norm
 
 
@@ -37,9 +65,10 @@ def static_density(mtot_d, mrat_d, redz_d, par, cosmo):
     gmt = D.empty((M, Q, Z))
     par = np.ascontiguousarray(par, dtype=np.float64)
     assert par.size == _lib.HD_DENS_NPAR
-    _lib.call("hd_static_density", D.ptr(mtot_d), M, D.ptr(mrat_d), Q, D.ptr(redz_d), Z, par.ctypes.data,
-              D.ptr(tabs["dc_tab"]), D.ptr(tabs["dc_der"]), tabs["n_tab"], D.ptr(dens), D.ptr(zp), D.ptr(gmt),
-              D.stream_ptr())
+    with _stage("density"):
+        _lib.call("hd_static_density", D.ptr(mtot_d), M, D.ptr(mrat_d), Q, D.ptr(redz_d), Z, par.ctypes.data,
+                  D.ptr(tabs["dc_tab"]), D.ptr(tabs["dc_der"]), tabs["n_tab"], D.ptr(dens), D.ptr(zp), D.ptr(gmt),
+                  D.stream_ptr())
     return dens, zp, gmt
 
 
@@ -53,9 +82,10 @@ def dbn_gw(fobs_orb_d, nden, mtot_d, mrat_d, redz_prime, cosmo):
     tabs = cosmo.device_tables()
     rf = D.empty((M, Q, Z, F))
     dn = D.empty((M, Q, Z, F))
-    _lib.call("hd_dbn_gw", D.ptr(fobs_orb_d), F, D.ptr(nden), D.ptr(mtot_d), M, D.ptr(mrat_d), Q, Z,
-              D.ptr(redz_prime), D.ptr(tabs["grid_z"]), D.ptr(tabs["grid_dcom"]), cosmo.size, D.ptr(rf), D.ptr(dn),
-              D.stream_ptr())
+    with _stage("dbn_gw"):
+        _lib.call("hd_dbn_gw", D.ptr(fobs_orb_d), F, D.ptr(nden), D.ptr(mtot_d), M, D.ptr(mrat_d), Q, Z,
+                  D.ptr(redz_prime), D.ptr(tabs["grid_z"]), D.ptr(tabs["grid_dcom"]), cosmo.size, D.ptr(rf), D.ptr(dn),
+                  D.stream_ptr())
     return rf, dn
 
 
@@ -66,18 +96,20 @@ def dbn_2pwl(fobs_orb_d, fobs_sorted, sepa_init, num_steps, norm_d, rchar, gamma
     tabs = cosmo.device_tables()
     rf = D.empty((M, Q, Z, F))
     dn = D.empty((M, Q, Z, F))
-    _lib.call("hd_dbn_2pwl", D.ptr(fobs_orb_d), F, int(bool(fobs_sorted)), float(sepa_init), int(num_steps),
-              D.ptr(norm_d), float(rchar), float(gamma_inner), float(gamma_outer), D.ptr(nden), D.ptr(mtot_d), M,
-              D.ptr(mrat_d), Q, D.ptr(redz_d), Z, D.ptr(gmt_time), D.ptr(tabs["grid_z"]), D.ptr(tabs["grid_dcom"]),
-              D.ptr(tabs["grid_age"]), cosmo.size, D.ptr(rf), D.ptr(dn), D.stream_ptr())
+    with _stage("dbn_2pwl"):
+        _lib.call("hd_dbn_2pwl", D.ptr(fobs_orb_d), F, int(bool(fobs_sorted)), float(sepa_init), int(num_steps),
+                  D.ptr(norm_d), float(rchar), float(gamma_inner), float(gamma_outer), D.ptr(nden), D.ptr(mtot_d), M,
+                  D.ptr(mrat_d), Q, D.ptr(redz_d), Z, D.ptr(gmt_time), D.ptr(tabs["grid_z"]), D.ptr(tabs["grid_dcom"]),
+                  D.ptr(tabs["grid_age"]), cosmo.size, D.ptr(rf), D.ptr(dn), D.stream_ptr())
     return rf, dn
 
 
 def integrate_3dx1d(log10_mtot_d, mrat_d, redz_d, dlnf_d, dnum):
     M, Q, Z, F = dnum.shape
     out = D.empty((M - 1, Q - 1, Z - 1, F))
-    _lib.call("hd_integrate_3dx1d", D.ptr(log10_mtot_d), M, D.ptr(mrat_d), Q, D.ptr(redz_d), Z, D.ptr(dlnf_d), F,
-              D.ptr(dnum), D.ptr(out), D.stream_ptr())
+    with _stage("integrate"):
+        _lib.call("hd_integrate_3dx1d", D.ptr(log10_mtot_d), M, D.ptr(mrat_d), Q, D.ptr(redz_d), Z, D.ptr(dlnf_d), F,
+                  D.ptr(dnum), D.ptr(out), D.stream_ptr())
     return out
 
 
@@ -85,9 +117,10 @@ def h2fdf_redz(mt_cent_d, mr_cent_d, fc_d, fcdf_d, redz_final, cosmo):
     M, Q, Z, F = redz_final.shape
     tabs = cosmo.device_tables()
     out = D.empty((M - 1, Q - 1, Z - 1, F))
-    _lib.call("hd_h2fdf_redz", D.ptr(mt_cent_d), M, D.ptr(mr_cent_d), Q, Z, D.ptr(fc_d), D.ptr(fcdf_d), F,
-              D.ptr(redz_final), float(GW_SRC_CONST), cosmo.Om0, cosmo.h, D.ptr(tabs["dc_tab"]),
-              D.ptr(tabs["dc_der"]), tabs["n_tab"], D.ptr(out), D.stream_ptr())
+    with _stage("h2fdf"):
+        _lib.call("hd_h2fdf_redz", D.ptr(mt_cent_d), M, D.ptr(mr_cent_d), Q, Z, D.ptr(fc_d), D.ptr(fcdf_d), F,
+                  D.ptr(redz_final), float(GW_SRC_CONST), cosmo.Om0, cosmo.h, D.ptr(tabs["dc_tab"]),
+                  D.ptr(tabs["dc_der"]), tabs["n_tab"], D.ptr(out), D.stream_ptr())
     return out
 
 
@@ -96,8 +129,9 @@ def order_desc(key2d, col=0):
     B, stride = key2d.shape
     order = D.empty(B, dtype=torch.int64)
     ws = D.workspace(_lib.size("hd_order_workspace_size", B))
-    _lib.call("hd_order_desc", D.ptr(key2d), B, stride, int(col), D.ptr(order), D.ptr(ws), ws.numel(),
-              D.stream_ptr())
+    with _stage("order"):
+        _lib.call("hd_order_desc", D.ptr(key2d), B, stride, int(col), D.ptr(order), D.ptr(ws), ws.numel(),
+                  D.stream_ptr())
     return order
 
 
@@ -112,16 +146,17 @@ def realize(number2d, h2fdf2d, nreals, nloud, mode, order=None, counts=None, fla
     hc2ss = D.empty((F, R, max(L, 1)))
     ssidx = D.empty((F, R, max(L, 1)), dtype=torch.int64)
     bgpar = D.empty((npar, F, R)) if npar else None
-    if counts is not None:
-        ws = D.workspace(_lib.size("hd_realize_counts_workspace_size", B, F, R, L, npar))
-        _lib.call("hd_realize_counts", D.ptr(counts), D.ptr(h2fdf2d), D.ptr(order), B, F, R, L, mode, flags,
-                  D.ptr(par), npar, D.ptr(hc2ss), D.ptr(ssidx), D.ptr(hc2bg), D.ptr(bgpar), D.ptr(ws), ws.numel(),
-                  D.stream_ptr())
-    else:
-        ws = D.workspace(_lib.size("hd_realize_workspace_size", B, F, R, L, npar))
-        _lib.call("hd_realize", D.ptr(number2d), D.ptr(h2fdf2d), D.ptr(order), B, F, R, L, mode, flags,
-                  float(normal_threshold), int(seed) & 0xFFFFFFFFFFFFFFFF, D.ptr(par), npar, D.ptr(hc2ss),
-                  D.ptr(ssidx), D.ptr(hc2bg), D.ptr(bgpar), D.ptr(ws), ws.numel(), D.stream_ptr())
+    with _stage("realize"):
+        if counts is not None:
+            ws = D.workspace(_lib.size("hd_realize_counts_workspace_size", B, F, R, L, npar))
+            _lib.call("hd_realize_counts", D.ptr(counts), D.ptr(h2fdf2d), D.ptr(order), B, F, R, L, mode, flags,
+                      D.ptr(par), npar, D.ptr(hc2ss), D.ptr(ssidx), D.ptr(hc2bg), D.ptr(bgpar), D.ptr(ws), ws.numel(),
+                      D.stream_ptr())
+        else:
+            ws = D.workspace(_lib.size("hd_realize_workspace_size", B, F, R, L, npar))
+            _lib.call("hd_realize", D.ptr(number2d), D.ptr(h2fdf2d), D.ptr(order), B, F, R, L, mode, flags,
+                      float(normal_threshold), int(seed) & 0xFFFFFFFFFFFFFFFF, D.ptr(par), npar, D.ptr(hc2ss),
+                      D.ptr(ssidx), D.ptr(hc2bg), D.ptr(bgpar), D.ptr(ws), ws.numel(), D.stream_ptr())
     if L == 0:
         hc2ss = hc2ss[..., :0]
         ssidx = ssidx[..., :0]

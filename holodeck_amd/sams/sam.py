@@ -130,6 +130,7 @@ class Semi_Analytic_Model:
         if self._density is None:
             dens, _, _ = self._density_device()
             self._density = D.to_host(dens)
+            self._check_scatter()
         return self._density
 
     @property
@@ -204,9 +205,31 @@ class Semi_Analytic_Model:
                              "models!")
         number, h2fdf, rf = self._number_h2fdf_device(fobs_gw_edges, hard)
         edges = [self.mtot, self.mrat, self.redz, np.asarray(fobs_gw_edges, dtype=float) / 2.0]
-        return single_sources._ss_gws_redz_device(edges, rf, number, h2fdf, realize=realize, loudest=loudest,
-                                                  params=params, seed=seed, counts=counts, loud_mode=loud_mode,
-                                                  cosmo=self._cosmo)
+        out = single_sources._ss_gws_redz_device(edges, rf, number, h2fdf, realize=realize, loudest=loudest,
+                                                 params=params, seed=seed, counts=counts, loud_mode=loud_mode,
+                                                 cosmo=self._cosmo)
+        self._check_scatter()
+        return out
+
+    def gwb_device(self, fobs_gw_edges, hard, realize, loudest, seed=0):
+        """Device-resident gwb: (hc2ss (F,R,L), ssidx (F,R,L), hc2bg (F,R)) tensors, no host copies.
+
+        Same pipeline as ``gwb(..., params=False)`` (frequency-0 order walk), for
+        callers that keep results in HBM (batch drivers, benchmarks)."""
+        from .. import engine
+        number, h2fdf, rf = self._number_h2fdf_device(fobs_gw_edges, hard)
+        order = engine.order_desc(h2fdf, col=0)
+        hc2ss, ssidx, hc2bg, _ = engine.realize(number, h2fdf, int(realize), int(loudest), _lib.HD_SEL_SORTED,
+                                                order=order, seed=seed)
+        return hc2ss, ssidx, hc2bg
+
+    def _check_scatter(self):
+        """Raise as the reference does (sam.py:1377-1383) if the nearest-node fill left bad values."""
+        it = self._scatter_iters
+        if it is not None:
+            bad = int(it[-1].item())
+            if bad:
+                raise ValueError(f"After 0th order interpolation, {bad} bad values remain!")
 
     def gwb_ideal(self, fobs_gw, sum=True, redz_prime=None):
         raise NotImplementedError("gwb_ideal is not part of the device path")

@@ -1,0 +1,192 @@
+"""M-Mbulge scatter of the binary density on the device.
+
+<- add_scatter_to_masses (holodeck/sams/sam.py:1291-1394) with
+   utils._get_rolled_weights / get_scatter_weights / _scatter_with_weights
+   (holodeck/utils.py:416-488).
+
+Per redshift slice the reference (i) Clough-Tocher-interpolates the density
+from the (log10 m1, log10 m2) nodes of the (M, q) grid onto a regular
+(4M)x(4M) log-mass grid, (ii) replaces NaN / negative values with the nearest
+node's value, (iii) convolves with Gaussian-CDF weights, (iv) interpolates
+bilinearly back to the (M, q) nodes.
+
+Note on (iii): ``_scatter_with_weights`` returns its einsum result with the
+convolved axis moved last, so the reference's two calls (axis=0, then axis=1)
+both act on the m1 axis; the slice result is (W W)^T D (D indexed [m1, m2]).
+This is reproduced as is.
+
+Split of work:
+* the PLAN -- geometry only, built once per (mtot, mrat, scatter) on the host:
+  the Delaunay triangulation of the nodes (scipy.spatial.Delaunay / Qhull, as
+  the reference's interpolator builds it), vertex adjacency in Qhull's order,
+  a level schedule of the Gauss-Seidel sweep, the simplex and barycentric
+  coordinates of every regular-grid point, each simplex's cross-boundary
+  derivative directions, the nearest node of every regular-grid point, the
+  Gaussian-CDF weight matrix and the bilinear weights back to the nodes;
+* the DEVICE -- everything that depends on the density: the global gradient
+  estimation (Gauss-Seidel, level-scheduled, identical update sequence), the
+  Clough-Tocher evaluation and bad-value fill, the FP64 GEMM with (W W)^T and
+  the bilinear gather, for all redshift slices at once.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as D
+
+_PLANS = {}
+
+
+def _rolled_weights(log_cents, scatter):
+    """utils._get_rolled_weights (utils.py:464-488) for a Normal(0, scatter) CDF."""
+    from scipy.stats import norm
+    num = log_cents.size
+    dx = np.diff(log_cents)
+    if not np.allclose(dx, dx[0]):
+        raise ValueError("`get_scatter_weights` only works if `uniform_cents` are uniformly spaced!")
+    dx = dx[0]
+    dx = dx / 2.0 + np.arange(num) * dx
+    dx = np.concatenate([-dx[::-1], dx])
+    dm = np.diff(norm(loc=0.0, scale=scatter).cdf(dx))          # (2N-1,)
+    # row j holds the weights from bin j to bins k: dm[k - j + N - 1]
+    j = np.arange(num)[:, None]
+    k = np.arange(num)[None, :]
+    return dm[k - j + num - 1]
+
+
+def _simplex_g(tri):
+    """Affine-invariant cross-boundary directions of each triangle (Clough-Tocher; see
+    P. Alfeld, CAGD 1 (1984) 169; G. Farin, CAGD 3 (1986) 83): for edge k (opposite
+    vertex k) the direction is V4' - V4, the neighbour's centroid minus this
+    centroid, expressed through the barycentric coordinates c of V4'."""
+    S = tri.simplices
+    P = tri.points
+    nb = tri.neighbors
+    T = tri.transform
+    g = np.full((S.shape[0], 3), -0.5)
+    cent = P[S].mean(axis=1)
+    for k in range(3):
+        it = nb[:, k]
+        ok = it >= 0
+        y = cent[it[ok]]
+        Tm = T[ok, :2, :]
+        r = T[ok, 2, :]
+        c01 = np.einsum('nij,nj->ni', Tm, y - r)
+        c = np.concatenate([c01, 1.0 - c01.sum(axis=1, keepdims=True)], axis=1)
+        if k == 0:
+            gk = (2 * c[:, 2] + c[:, 1] - 1) / (2 - 3 * c[:, 2] - 3 * c[:, 1])
+        elif k == 1:
+            gk = (2 * c[:, 0] + c[:, 2] - 1) / (2 - 3 * c[:, 0] - 3 * c[:, 2])
+        else:
+            gk = (2 * c[:, 1] + c[:, 0] - 1) / (2 - 3 * c[:, 1] - 3 * c[:, 0])
+        g[ok, k] = gk
+    return g
+
+
+class ScatterPlan:
+    """Geometry of the scatter step for one (mtot, mrat, scatter) triple."""
+
+    def __init__(self, mtot, mrat, scatter, refine=4):
+        from scipy.spatial import Delaunay, cKDTree
+        mtot = np.asarray(mtot, dtype=float)
+        mrat = np.asarray(mrat, dtype=float)
+        M, Q = mtot.size, mrat.size
+        m1 = mtot[:, None] / (1.0 + mrat[None, :])
+        m2 = mtot[:, None] - m1
+        pts = np.stack([np.log10(m1.ravel()), np.log10(m2.ravel())], axis=-1)
+        G = M * refine
+        lo = 0.9 * mtot[0] * mrat[0] / (1.0 + mrat[0])
+        hi = mtot[-1] * (1.0 + mrat[0]) / mrat[0]
+        mextr = np.array([min(lo, hi), max(lo, hi)])
+        mgrid = np.log10(np.logspace(*np.log10(mextr), G))
+        tri = Delaunay(pts)
+        indptr, indices = tri.vertex_neighbor_vertices
+        N = pts.shape[0]
+        # level schedule: a vertex may update once all lower-indexed neighbours have
+        level = np.zeros(N, dtype=np.int64)
+        for p in range(N):
+            nb = indices[indptr[p]:indptr[p + 1]]
+            lo_nb = nb[nb < p]
+            level[p] = (level[lo_nb].max() + 1) if lo_nb.size else 0
+        nlev = int(level.max()) + 1
+        lorder = np.argsort(level, kind='stable')
+        lptr = np.concatenate([[0], np.cumsum(np.bincount(level, minlength=nlev))])
+        # regular-grid query points xi[i*G + j] = (mgrid[i], mgrid[j])
+        X, Y = np.meshgrid(mgrid, mgrid, indexing='ij')
+        xi = np.stack([X.ravel(), Y.ravel()], axis=-1)
+        simp = tri.find_simplex(xi)
+        bary = np.zeros((xi.shape[0], 3))
+        ok = simp >= 0
+        Tm = tri.transform[simp[ok], :2, :]
+        r = tri.transform[simp[ok], 2, :]
+        c01 = np.einsum('nij,nj->ni', Tm, xi[ok] - r)
+        bary[ok, :2] = c01
+        bary[ok, 2] = 1.0 - c01.sum(axis=1)
+        nearest = cKDTree(pts).query(xi)[1]
+        W = _rolled_weights(mgrid, scatter)
+        # bilinear back-interpolation (RegularGridInterpolator 'linear') at the nodes
+        ii = np.clip(np.searchsorted(mgrid, pts[:, 0]) - 1, 0, G - 2)
+        jj = np.clip(np.searchsorted(mgrid, pts[:, 1]) - 1, 0, G - 2)
+        wx = (pts[:, 0] - mgrid[ii]) / (mgrid[ii + 1] - mgrid[ii])
+        wy = (pts[:, 1] - mgrid[jj]) / (mgrid[jj + 1] - mgrid[jj])
+        back_idx = np.stack([ii * G + jj, ii * G + jj + 1, (ii + 1) * G + jj, (ii + 1) * G + jj + 1], axis=-1)
+        back_w = np.stack([(1 - wx) * (1 - wy), (1 - wx) * wy, wx * (1 - wy), wx * wy], axis=-1)
+
+        self.M, self.Q, self.G, self.N = M, Q, G, N
+        self.n_levels = nlev
+        self.host = dict(
+            pts=pts, nbr_ptr=indptr.astype(np.int32), nbr_idx=indices.astype(np.int32),
+            levels_ptr=lptr.astype(np.int32), levels_idx=lorder.astype(np.int32),
+            simplices=tri.simplices.astype(np.int32), simplex_g=_simplex_g(tri), q_simplex=simp.astype(np.int32),
+            q_bary=bary, q_nearest=nearest.astype(np.int32), wT=np.ascontiguousarray(W.T),
+            back_idx=back_idx.astype(np.int32), back_w=back_w)
+        self.tri = tri
+        self.mgrid = mgrid
+        self.W = W
+        self._dev = None
+
+    def device(self):
+        if self._dev is None:
+            h = self.host
+            d = {}
+            for k, v in h.items():
+                d[k] = D.to_dev(v, dtype=torch.int32 if v.dtype == np.int32 else torch.float64)
+            # P = (W W)^T = W^T W^T on the device
+            G = self.G
+            P = D.empty((G, G))
+            _lib.call("hd_gemm_f64", D.ptr(d['wT']), D.ptr(d['wT']), D.ptr(P), G, G, G, D.stream_ptr())
+            d['P'] = P
+            self._dev = d
+        return self._dev
+
+
+def get_plan(mtot, mrat, scatter):
+    key = (np.asarray(mtot).tobytes(), np.asarray(mrat).tobytes(), float(scatter))
+    plan = _PLANS.get(key)
+    if plan is None:
+        plan = ScatterPlan(mtot, mrat, scatter)
+        _PLANS[key] = plan
+    return plan
+
+
+def add_scatter_to_masses_device(mtot, mrat, dens, scatter, tol=1e-6, maxiter=400):
+    """In place on the (M, Q, Z) device density; returns the per-slice Gauss-Seidel sweep counts (0 = no
+    convergence within maxiter, as the reference's estimator reports)."""
+    plan = get_plan(mtot, mrat, scatter)
+    d = plan.device()
+    M, Q, Z = dens.shape
+    assert (M, Q) == (plan.M, plan.Q)
+    ws = D.workspace(_lib.size("hd_scatter_workspace_size", M, Q, Z, plan.G))
+    iters = D.empty(Z + 1, dtype=torch.int32)
+    from .engine import _stage
+    with _stage("scatter"):
+        _scatter_call(dens, M, Q, Z, plan, d, tol, maxiter, ws, iters)
+    return iters
+
+
+def _scatter_call(dens, M, Q, Z, plan, d, tol, maxiter, ws, iters):
+    _lib.call("hd_scatter", D.ptr(dens), M, Q, Z, plan.G, D.ptr(d['pts']), D.ptr(d['nbr_ptr']), D.ptr(d['nbr_idx']),
+              D.ptr(d['levels_ptr']), D.ptr(d['levels_idx']), plan.n_levels, D.ptr(d['simplices']),
+              D.ptr(d['simplex_g']), D.ptr(d['q_simplex']), D.ptr(d['q_bary']), D.ptr(d['q_nearest']),
+              D.ptr(d['P']), D.ptr(d['back_idx']), D.ptr(d['back_w']), float(tol), int(maxiter), D.ptr(ws),
+              ws.numel(), D.ptr(iters), D.stream_ptr())

@@ -65,6 +65,32 @@ int hd_static_density(const double *mtot, int M, const double *mrat, int Q, cons
 int hd_zero_stalled(double *dens, const double *zprime, int64_t n, void *stream);
 
 /* ------------------------------------------------------------------------
+ * M-Mbulge scatter  <- add_scatter_to_masses (sam.py:1291-1394) with
+ * utils._get_rolled_weights/_scatter_with_weights (utils.py:416-488).
+ * dens (M,Q,Z) is updated in place, all Z slices at once.  The plan arrays
+ * are geometry built once per grid shape on the host (holodeck_amd/scatter.py):
+ *   pts (N,2) node coordinates (log10 m1, log10 m2), N = M*Q, C order;
+ *   nbr_ptr (N+1) / nbr_idx: vertex adjacency of the Delaunay triangulation;
+ *   levels_ptr (n_levels+1) / levels_idx (N): Gauss-Seidel level schedule;
+ *   simplices (S,3), simplex_g (S,3) cross-boundary derivative directions;
+ *   q_simplex (G*G) (-1 outside the hull), q_bary (G*G,3), q_nearest (G*G);
+ *   P (G,G) = (W W)^T for the Gaussian-CDF weight matrix W (see hd_gemm_f64);
+ *   back_idx (N,4) regular-grid points, back_w (N,4) bilinear weights.
+ * iters_out: Z+1 int32 -- Gauss-Seidel sweeps per slice (0: no convergence
+ * within maxiter) and, last, the count of values still NaN/negative after the
+ * nearest-node fill (the reference raises in that case, sam.py:1377-1383).
+ * ---------------------------------------------------------------------- */
+size_t hd_scatter_workspace_size(int M, int Q, int Z, int G);
+int hd_scatter(double *dens, int M, int Q, int Z, int G, const double *pts, const int32_t *nbr_ptr,
+               const int32_t *nbr_idx, const int32_t *levels_ptr, const int32_t *levels_idx, int n_levels,
+               const int32_t *simplices, const double *simplex_g, const int32_t *q_simplex, const double *q_bary,
+               const int32_t *q_nearest, const double *P, const int32_t *back_idx, const double *back_w, double tol,
+               int maxiter, void *workspace, size_t ws_bytes, int32_t *iters_out, void *stream);
+/* C (m,n) = A (m,k) B (k,n), row-major FP64 (the scatter's convolution GEMM;
+ * the reference evaluates it with numpy.einsum, utils.py:455-461). */
+int hd_gemm_f64(const double *A, const double *B, double *C, int m, int n, int k, void *stream);
+
+/* ------------------------------------------------------------------------
  * Binary evolution  <- holodeck/sams/sam_cyutils.pyx
  * ---------------------------------------------------------------------- */
 /* find_2pwl_hardening_norm (sam_cyutils.pyx:289-353): brentq per (mtot,mrat).

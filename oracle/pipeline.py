@@ -377,7 +377,10 @@ def number_h2fdf(cfg, density=None):
     mtot, mrat, redz = edges_from_cfg(cfg)
     dens, zprime, gmt = static_density(mtot, mrat, redz, cfg, cosmo)
     if density is not None:
-        dens = density          # e.g. the scatter oracle's output, or an injected density
+        dens = density          # an injected density
+    elif cfg['mmbulge'].get('scatter_dex', 0.0) > 0.0:
+        from .scatter import add_scatter_to_masses
+        dens = add_scatter_to_masses(mtot, mrat, dens, cfg['mmbulge']['scatter_dex'])
     if zprime is not None:
         dens = np.array(dens)
         dens[zprime < 0.0] = 0.0

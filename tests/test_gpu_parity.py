@@ -132,3 +132,43 @@ def test_pipeline_counts_end_to_end(gpu, oracle_lib):
     hc_ss, hc_bg = sam.gwb(configs.fobs_gw_edges(spec), hard=hard, realize=R, loudest=L, counts=counts)
     _close(hc_bg, np.sqrt(bg_o), 1e-5, what="hc_bg")
     _close(hc_ss, np.sqrt(ss_o), 1e-5, what="hc_ss")
+
+
+def test_scatter_and_density_full_config1(gpu, oracle_lib):
+    """Density with M-Mbulge scatter at the full 61x51x41 grid vs the oracle's scipy-based restatement."""
+    print("[scatter full config1]", flush=True)
+    from oracle.cfg import oracle_cfg
+    from oracle.scatter import add_scatter_to_masses
+    from holodeck_amd import device as D
+    spec = configs.spec_config1()
+    sam = configs.make_sam(spec)
+    dens = sam.static_binary_density
+    iters = D.to_host(sam._scatter_iters)
+    assert np.all(iters[:-1] > 0) and iters[-1] == 0, iters
+    cfg = oracle_cfg(spec)
+    c = cfg['cosmo']
+    cos = oracle_lib.OracleCosmo(h=c['h'], Om0=c['Om0'])
+    mtot, mrat, redz = oracle_lib.edges_from_cfg(cfg)
+    d0, zp, _ = oracle_lib.static_density(mtot, mrat, redz, cfg, cos)
+    do = add_scatter_to_masses(mtot, mrat, d0, cfg['mmbulge']['scatter_dex'])
+    do[zp < 0] = 0.0
+    scale = np.max(np.abs(do))
+    _close(dens, do, 1e-8, atol=1e-13 * scale, what="scattered density")
+
+
+def test_stageA_full_config1_with_scatter(gpu, oracle_lib):
+    """number and h2fdf at the full config-1 workload (61x51x41, 20 frequencies, GW-only)."""
+    print("[stageA full config1]", flush=True)
+    from oracle.cfg import oracle_cfg
+    from holodeck_amd import device as D
+    spec = configs.spec_config1()
+    sam = configs.make_sam(spec)
+    hard = configs.make_hard(spec, sam)
+    o = oracle_lib.number_h2fdf(oracle_cfg(spec))
+    number, h2fdf, rf = sam._number_h2fdf_device(configs.fobs_gw_edges(spec), hard)
+    F = number.shape[-1]
+    n_o = o['number'].reshape(-1, F)
+    _close(D.to_host(rf), o['redz_final'], 1e-9, what="redz_final")
+    _close(D.to_host(number), n_o, 1e-7, atol=1e-12 * n_o.max(), what="number")
+    h_o = o['h2fdf'].reshape(-1, F)
+    _close(D.to_host(h2fdf), h_o, 1e-9, what="h2fdf")

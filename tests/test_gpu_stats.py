@@ -1,0 +1,100 @@
+"""Statistics of the on-GPU counter-based realizations.
+
+* the per-cell sampler against the Poisson (and, above 1e10, Normal) law:
+  moments and a chi-square / KS test per lambda regime;
+* the realized h_c (background and loudest) against the CPU oracle's own
+  independent generator (xoshiro256** + the numpy-documented samplers):
+  per-frequency two-sample KS p > 0.01 and median / 5th / 95th percentiles
+  within Monte-Carlo error.  The reference's draws are unseeded
+  (cyutils.pyx:1313-1317), so agreement is distributional.
+"""
+import numpy as np
+import pytest
+from scipy import stats
+
+from holodeck_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _draws(lam, R, seed):
+    from holodeck_amd import engine, _lib
+    from holodeck_amd import device as D
+    n = D.to_dev(np.array([[lam]]))
+    h = D.to_dev(np.array([[1.0]]))
+    _, _, bg, _ = engine.realize(n, h, R, 0, _lib.HD_SEL_NONE, seed=seed)
+    return D.to_host(bg)[0]
+
+
+@pytest.mark.parametrize("lam", [1e-7, 0.02, 0.7, 3.0, 9.99, 10.0, 47.5, 1.0e3, 3.3e6])
+def test_poisson_sampler(lam, gpu):
+    print(f"[poisson lam={lam}]", flush=True)
+    R = 200000
+    x = _draws(lam, R, seed=17)
+    assert np.all(x == np.floor(x)) and np.all(x >= 0)
+    se = np.sqrt(lam / R)
+    assert abs(x.mean() - lam) < 5 * se + 1e-12
+    if lam > 1e-3:
+        assert abs(x.var() / lam - 1) < 5 * np.sqrt(2.0 / R) + 5 / np.sqrt(R * lam)
+    # chi-square over the bulk of the pmf
+    kmax = int(stats.poisson.ppf(1 - 1e-4, lam))
+    kmin = int(stats.poisson.ppf(1e-4, lam))
+    ks = np.arange(kmin, kmax + 1)
+    obs = np.array([np.sum(x == k) for k in ks], dtype=float)
+    exp = stats.poisson.pmf(ks, lam) * R
+    ok = exp > 20
+    if ok.sum() >= 2:
+        o, e = obs[ok], exp[ok]
+        chi2 = np.sum((o - e) ** 2 / e)
+        p = stats.chi2.sf(chi2, ok.sum() - 1)
+        assert p > 1e-3, (lam, chi2, ok.sum(), p)
+    if lam < 1e-5:
+        assert np.sum(x > 0) <= 5
+
+
+def test_normal_branch_above_threshold(gpu):
+    lam = 3.0e10
+    x = _draws(lam, 100000, seed=5)
+    z = (x - lam) / np.sqrt(lam)
+    assert abs(z.mean()) < 0.02 and abs(z.std() - 1) < 0.02
+    assert stats.kstest(z, 'norm').pvalue > 1e-3
+    assert np.any(x != np.floor(x))   # not floored, as the reference
+
+
+def test_streams_independent_of_launch_and_seeded(gpu):
+    a = _draws(2.5, 5000, seed=3)
+    b = _draws(2.5, 5000, seed=3)
+    c = _draws(2.5, 5000, seed=4)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    # a prefix of the realizations does not depend on R (counter keyed by r)
+    d = _draws(2.5, 777, seed=3)
+    assert np.array_equal(a[:777], d)
+
+
+def test_hc_distribution_vs_cpu_oracle(gpu, oracle_lib):
+    """Realized hc_bg and loudest hc_ss: GPU generator vs the oracle's generator, same number/h2fdf."""
+    print("[hc distribution vs oracle]", flush=True)
+    from oracle.cfg import oracle_cfg
+    from holodeck_amd import engine, _lib
+    from holodeck_amd import device as D
+    spec = configs.spec_config1(shape=(21, 17, 15), nfreq=5)
+    o = oracle_lib.number_h2fdf(oracle_cfg(spec))
+    F = o['number'].shape[-1]
+    n2 = o['number'].reshape(-1, F)
+    h2 = o['h2fdf'].reshape(-1, F)
+    order = oracle_lib.order_desc(h2[:, 0])
+    R, L = 1000, 3
+    ss_c, bg_c = oracle_lib.loudest_rng(n2, h2, order, R, L, seed=99)
+    ss_g, _, bg_g, _ = engine.realize(D.to_dev(n2), D.to_dev(h2), R, L, _lib.HD_SEL_SORTED, order=D.to_dev(order),
+                                      seed=2024)
+    ss_g, bg_g = D.to_host(ss_g), D.to_host(bg_g)
+    for f in range(F):
+        for name, a, b in (("bg", bg_g[f], bg_c[f]), ("ss0", ss_g[f, :, 0], ss_c[f, :, 0])):
+            p = stats.ks_2samp(np.log10(a + 1e-300), np.log10(b + 1e-300)).pvalue
+            assert p > 0.01, (name, f, p)
+            for q in (5, 50, 95):
+                qa, qb = np.percentile(a, q), np.percentile(b, q)
+                # Monte-Carlo error of a percentile ~ sqrt(q(1-q)/R)/pdf; use the pooled spread
+                spread = np.percentile(np.concatenate([a, b]), [q - 3, q + 3]) if 3 <= q <= 97 else None
+                if spread is not None:
+                    assert spread[0] - 1e-300 <= qa <= spread[1] + 1e-300 or abs(qa - qb) <= (spread[1] - spread[0])
