@@ -333,12 +333,47 @@ static int realize_impl(bool injected, const double* number, const double* count
     int threads = (int)min((int64_t)512, cdiv(R, 64) * 64);
     dim3 grid(A.nchunks, F);
     if (injected) hipLaunchKernelGGL(k_realize_chunk<true>, grid, dim3(threads), 0, st, A);
-    else hipLaunchKernelGGL(k_realize_chunk<false>, grid, dim3(threads), 0, st, A);
+    else return set_error(HD_ERR_ARG, "realize: the drawn-count path is hd_realize");
     int rc = check_launch("k_realize_chunk");
     if (rc) return rc;
     hipLaunchKernelGGL(k_realize_merge, dim3((unsigned)cdiv((int64_t)F * R, 256)), dim3(256), 0, st, A, hc2ss, ssidx,
                        hc2bg, (npar > 0) ? bgpar : nullptr);
     return check_launch("k_realize_merge");
+}
+
+// Merge of per-chunk partials, shared with the drawn-count kernel (realize_gap.hip).
+int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int F, int R, int L, int mode, int npar,
+                         int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
+                         const int32_t* p_bin, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
+                         hipStream_t st) {
+    RealizeArgs A;
+    A.number = nullptr;
+    A.counts = nullptr;
+    A.h2fdf = h2fdf;
+    A.rank = nullptr;
+    A.par = par;
+    A.B = B;
+    A.F = F;
+    A.R = R;
+    A.L = L;
+    A.mode = mode;
+    A.flags = 0;
+    A.npar = npar;
+    A.nchunks = nchunks;
+    A.thresh = 0.0;
+    A.seed = 0;
+    A.p_total = const_cast<double*>(p_total);
+    A.p_ptot = const_cast<double*>(p_ptot);
+    A.p_key = const_cast<uint64_t*>(p_key);
+    A.p_bin = const_cast<int32_t*>(p_bin);
+    hipLaunchKernelGGL(k_realize_merge, dim3((unsigned)cdiv((int64_t)F * R, 256)), dim3(256), 0, st, A, hc2ss, ssidx,
+                       hc2bg, bgpar);
+    return check_launch("k_realize_merge");
+}
+
+int rank_from_order_launch(const int64_t* order, int64_t B, int32_t* rank, hipStream_t st) {
+    hipLaunchKernelGGL(k_rank_from_order, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, st, order, B, rank);
+    return check_launch("k_rank_from_order");
 }
 
 }  // namespace hd
@@ -355,18 +390,6 @@ extern "C" int hd_realize_counts(const double* counts, const double* h2fdf, cons
                                  void* stream) {
     return realize_impl(true, nullptr, counts, h2fdf, order, B, F, R, L, mode, flags, 0.0, 0, par, npar, hc2ss, ssidx,
                         hc2bg, bgpar, workspace, ws_bytes, (hipStream_t)stream);
-}
-
-extern "C" size_t hd_realize_workspace_size(int64_t B, int F, int R, int L, int npar) {
-    return realize_ws(B, F, R, L, npar);
-}
-
-extern "C" int hd_realize(const double* number, const double* h2fdf, const int64_t* order, int64_t B, int F, int R, int L,
-                          int mode, int flags, double normal_threshold, uint64_t seed, const double* par, int npar,
-                          double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar, void* workspace,
-                          size_t ws_bytes, void* stream) {
-    return realize_impl(false, number, nullptr, h2fdf, order, B, F, R, L, mode, flags, normal_threshold, seed, par, npar,
-                        hc2ss, ssidx, hc2bg, bgpar, workspace, ws_bytes, (hipStream_t)stream);
 }
 
 extern "C" void hd_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
