@@ -23,6 +23,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GWB spectra/sec on 1 MI355X: plain GPU SAM vs MLP surrogate at ≤0.05 dex err"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_hd_realize_91x81x101_f40_r500.json")
+CALIB_BYTES = 1 << 30   # copy-kernel calibration size for the PMC passes (bytes read = bytes written)
 
 
 def cpu_baseline(spec, nz_sample=11, r_sample=20):
@@ -88,6 +90,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nreals", type=int, default=500)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed steps, copy a 1 GiB device buffer once (reference kernel for PMC byte "
+                         "counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -95,7 +100,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from holodeck_amd import configs, engine
+    from holodeck_amd import configs, engine, _lib
     from holodeck_amd.cosmology import Cosmology
 
     spec = configs.spec_config2()
@@ -115,6 +120,7 @@ def main():
         out = step(1000 + w)
     torch.cuda.synchronize()
     engine.EVENTS = {}
+    _lib.timing_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -123,14 +129,30 @@ def main():
     dt = time.perf_counter() - t0
     stages = engine.stage_times_ms()
     engine.EVENTS = None
+    real_ms, real_n = _lib.timing_read("hd_realize")
+    draw_ms, draw_n = _lib.timing_read("k_gz_draw")
+    _lib.timing_enable(False)
     hc2bg = out[2]
     assert torch.isfinite(hc2bg).all() and bool((hc2bg > 0).all())
+    if args.copy_calibration:
+        src = torch.empty(CALIB_BYTES // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ms = dt / args.steps * 1e3
     value = args.steps / dt
-    t_real = stages.get("realize", float("nan")) * 1e-3
+    # the realization launch sequence (rank, prep, draw, big-lambda draw, merge), timed by the
+    # library with HIP events on its launch stream
+    t_real = real_ms / max(real_n, 1) * 1e-3
     alg_bytes = 16.0 * B * F          # number + h2fdf (FP64) read once for all realizations
     achieved = alg_bytes / t_real / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("traffic_bytes_per_launch")
     line = {
         "metric": METRIC, "value": value, "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -142,9 +164,10 @@ def main():
                    "grid": [M, Q, Z], "nfreq": F, "nreals": R, "nloudest": L},
         "cell_realizations_per_s": B * F * R * value,
         "stages_ms": stages,
-        "roofline": {"bound": "hbm", "kernel": "realize", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": alg_bytes},
+        "roofline": {"bound": "hbm", "kernel": "hd_realize (k_gz_prep + k_gz_draw + k_gz_big + k_realize_merge)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "alg_bytes_per_launch": alg_bytes, "launch_ms": t_real * 1e3,
+                     "launches": real_n, "k_gz_draw_ms": draw_ms / max(draw_n, 1)},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec)

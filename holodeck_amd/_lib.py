@@ -24,6 +24,8 @@ _SZ = C.c_size_t
 SIGNATURES = {
     "hd_last_error": (C.c_char_p, []),
     "hd_version": (_I32, []),
+    "hd_timing_enable": (_I32, [_I32]),
+    "hd_timing_read": (_I32, [C.c_char_p, _P, _P]),
     "hd_cosmo_build_dcom_table": (_I32, [_D, _I32, _P, _P, _P]),
     "hd_cosmo_eval": (_I32, [_D, _D, _P, _P, _I32, _P, _I64, _P, _P, _P, _P]),
     "hd_static_density": (_I32, [_P, _I32, _P, _I32, _P, _I32, _P, _P, _P, _I32, _P, _P, _P, _P]),
@@ -107,3 +109,16 @@ def philox4x32_10(ctr, key):
     o = (C.c_uint32 * 4)()
     lib.hd_philox4x32_10(C.addressof(c), C.addressof(k), C.addressof(o))
     return [int(v) for v in o]
+
+
+def timing_enable(on=True):
+    """Start (clearing earlier records) or stop launch timing in the library."""
+    call("hd_timing_enable", 1 if on else 0)
+
+
+def timing_read(name):
+    """(total milliseconds, launches) recorded for one timed launch sequence since timing_enable."""
+    ms = C.c_double(0.0)
+    n = C.c_int(0)
+    call("hd_timing_read", name.encode(), C.addressof(ms), C.addressof(n))
+    return ms.value, n.value

@@ -20,6 +20,22 @@ namespace hd {
 int set_error(int code, const char* fmt, ...);
 int check_launch(const char* what);
 
+// Records HIP events around a launch sequence on its stream while timing is
+// enabled (hd_timing_enable); a no-op otherwise.
+class KTimer {
+  public:
+    KTimer(const char* name, hipStream_t st);
+    ~KTimer();
+    KTimer(const KTimer&) = delete;
+    KTimer& operator=(const KTimer&) = delete;
+
+  private:
+    const char* name_;
+    hipStream_t st_;
+    hipEvent_t a_, b_;
+    bool on_;
+};
+
 // ---- sam_cyutils.pyx:30-41 ----
 constexpr double MY_NWTG = 6.6742999e-08;
 constexpr double MY_SPLC = 29979245800.0;

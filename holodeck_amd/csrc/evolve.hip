@@ -196,34 +196,67 @@ __global__ void __launch_bounds__(256) k_dbn_gw(EvoConsts K, const double* __res
 // One workgroup per (M,Q) pair.  The workgroup first tabulates the evolution
 // steps in LDS (separation chain, rest-frame frequency, dt, cumulative time;
 // the chain and the cumulative sum are sequential exactly as in the
-// reference), then each thread owns one redshift corner kk and walks the
-// steps, bracketing the target frequencies.  A thread owns its whole output
-// row, so the reference's last-write-wins order per (kk, ff) is preserved.
+// reference).  Then
+//   phase 1: each thread owns one redshift corner kk and walks the steps,
+//            bracketing the target frequencies; for every target it records
+//            the LAST step that brackets it (the reference's last-write-wins
+//            per (kk, ff)), honouring the reference's break when the
+//            interpolated time passes the age of the universe.  The step index
+//            is parked in redz_final (the workgroup owns those rows);
+//   phase 2: one thread per (kk, ff) recomputes that step's bracket and does
+//            the per-target work (Kepler separation, hardening rate, residence
+//            time, cosmological factor) with every lane busy.
+// Table walks: the reference walks from the previous index (while_while_
+// increasing); the times walked are non-decreasing along the steps, so a walk
+// from the start of the table gives the same interval (the binary searches in
+// phase 2 compute exactly that interval).
 // ---------------------------------------------------------------------------
-constexpr int DBN_MAX_STEPS = 512;
-constexpr int DBN_MAX_TAB = 512;
+constexpr int DBN_MAX_STEPS = 4096;
+constexpr int DBN_THREADS = 128;
 
-__global__ void __launch_bounds__(128) k_dbn_2pwl(EvoConsts K, const double* __restrict__ fobs, int F, int sorted_f,
-                                                  double sepa_init, int nsteps, const double* __restrict__ hard_norm,
-                                                  double rchar, double g_in, double g_out,
-                                                  const double* __restrict__ nden, const double* __restrict__ mtot, int M,
-                                                  const double* __restrict__ mrat, int Q, const double* __restrict__ redz, int Z,
-                                                  const double* __restrict__ gmt_time,
-                                                  const double* __restrict__ gz_g, const double* __restrict__ gdc_g,
-                                                  const double* __restrict__ gage_g, int n_interp,
-                                                  double* __restrict__ redz_final, double* __restrict__ diff_num) {
-    __shared__ double s_frst[DBN_MAX_STEPS + 1];
-    __shared__ double s_dadt[DBN_MAX_STEPS + 1];
-    __shared__ double s_sepa[DBN_MAX_STEPS + 1];
-    __shared__ double s_dt[DBN_MAX_STEPS + 1];
-    __shared__ double s_tevo[DBN_MAX_STEPS + 1];
-    __shared__ double s_gz[DBN_MAX_TAB], s_gdc[DBN_MAX_TAB], s_gage[DBN_MAX_TAB];
+// first index i in [0, n-2] with edges[i+1] >= val (n-2 if none): the
+// interval a forward walk of while_while_increasing stops at
+__device__ __forceinline__ int bracket_increasing(int n, double val, const double* edges) {
+    int lo = 0, hi = n - 2;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (edges[mid + 1] < val) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const double* __restrict__ fobs, int F,
+                                                          int sorted_f, double sepa_init, int nsteps,
+                                                          const double* __restrict__ hard_norm, double rchar,
+                                                          double g_in, double g_out, const double* __restrict__ nden,
+                                                          const double* __restrict__ mtot, int M,
+                                                          const double* __restrict__ mrat, int Q,
+                                                          const double* __restrict__ redz, int Z,
+                                                          const double* __restrict__ gmt_time,
+                                                          const double* __restrict__ gz_g, const double* __restrict__ gdc_g,
+                                                          const double* __restrict__ gage_g, int n_interp,
+                                                          double* __restrict__ redz_final, double* __restrict__ diff_num) {
+    extern __shared__ double dbn_lds[];
+    const int ns1 = nsteps + 1;
+    double* s_frst = dbn_lds;
+    double* s_dt = s_frst + ns1;
+    double* s_tevo = s_dt + ns1;
+    double* s_sepa = s_tevo + ns1;     // set-up only
+    double* s_dadt = s_sepa + ns1;     // set-up only
+    double* s_gz = s_dadt + ns1;
+    double* s_gdc = s_gz + n_interp;
+    double* s_gage = s_gdc + n_interp;
+    double* s_agek = s_gage + n_interp;   // (Z)
 
     const int ii = blockIdx.x / Q;
     const int jj = blockIdx.x % Q;
     const int tid = threadIdx.x;
     const double mt = mtot[ii], mr = mrat[jj];
     const double norm = hard_norm[(int64_t)ii * Q + jj];
+    const int64_t row0 = ((int64_t)ii * Q + jj) * Z;     // first corner of this (M,Q) column
+    double* rf0 = redz_final + row0 * F;
+    double* dn0 = diff_num + row0 * F;
 
     for (int i = tid; i < n_interp; i += blockDim.x) {
         s_gz[i] = gz_g[i];
@@ -248,6 +281,14 @@ __global__ void __launch_bounds__(128) k_dbn_2pwl(EvoConsts K, const double* __r
         s_dadt[s] = hard_func_2pwl_gw(K, mt, mr, sp, norm, rchar, g_in, g_out);
         s_sepa[s] = sp;   // each s is read and written by the same thread
     }
+    // age at each redshift edge from the table (sam_cyutils.pyx:586-601)
+    for (int kk = tid; kk < Z; kk += blockDim.x) {
+        const double zk = redz[kk];
+        int ia = 0;
+        while ((ia + 1 < n_interp) && (gz_g[ia + 1] > zk)) ia++;
+        if (ia > n_interp - 2) ia = n_interp - 2;
+        s_agek[kk] = interp_between(zk, gz_g[ia], gz_g[ia + 1], gage_g[ia], gage_g[ia + 1]);
+    }
     __syncthreads();
     for (int s = tid + 1; s <= nsteps; s += blockDim.x)
         s_dt[s] = 2.0 * (s_sepa[s] - s_sepa[s - 1]) / (s_dadt[s - 1] + s_dadt[s]);
@@ -263,23 +304,12 @@ __global__ void __launch_bounds__(128) k_dbn_2pwl(EvoConsts K, const double* __r
     __syncthreads();
 
     const double age_universe = s_gage[n_interp - 1];
+    // phase 1: the bracketing step of every (kk, ff), parked in redz_final
     for (int kk = tid; kk < Z; kk += blockDim.x) {
-        const int64_t cell = ((int64_t)ii * Q + jj) * Z + kk;
-        double* rf = redz_final + cell * F;
-        double* dn = diff_num + cell * F;
-        for (int ff = 0; ff < F; ff++) {
-            rf[ff] = -1.0;
-            dn[ff] = 0.0;
-        }
-        // age at this redshift edge from the table (sam_cyutils.pyx:586-601)
-        const double zk = redz[kk];
-        int ia = 0;
-        while ((ia + 1 < n_interp) && (s_gz[ia + 1] > zk)) ia++;
-        if (ia > n_interp - 2) ia = n_interp - 2;
-        const double age_k = interp_at_index(ia, zk, s_gz, s_gage);
-        const double gmt = gmt_time[cell];
-        const double nd = nden[cell];
-
+        double* rf = rf0 + (int64_t)kk * F;
+        for (int ff = 0; ff < F; ff++) rf[ff] = -1.0;
+        const double age_k = s_agek[kk];
+        const double gmt = gmt_time[row0 + kk];
         int idx_left = 0;
         for (int step = 0; step < nsteps; step++) {
             const double dt = s_dt[step + 1];
@@ -294,7 +324,6 @@ __global__ void __launch_bounds__(128) k_dbn_2pwl(EvoConsts K, const double* __r
             if (redz_right < 0.0) redz_right = 0.0;
             const double fl = s_frst[step] / (1.0 + redz_left);
             const double fr = s_frst[step + 1] / (1.0 + redz_right);
-
             int ff = 0;
             if (sorted_f) {
                 // first target >= fl (targets are non-decreasing)
@@ -313,20 +342,47 @@ __global__ void __launch_bounds__(128) k_dbn_2pwl(EvoConsts K, const double* __r
                     continue;
                 }
                 const double new_time = interp_between(ft, fl, fr, time_left, time_right);
-                if (new_time > s_gage[n_interp - 1]) break;
-                const int inew = ww_increasing(idx_left, n_interp, new_time, s_gage);
-                const double new_redz = interp_at_index(inew, new_time, s_gage, s_gz);
-                const double dcom = interp_at_index(inew, new_time, s_gage, s_gdc);
-                rf[ff] = new_redz;
-                const double frst = ft * (1.0 + new_redz);
-                const double sepa = kepler_sepa_from_freq(K, mt, frst);
-                const double dadt = hard_func_2pwl_gw(K, mt, mr, sepa, norm, rchar, g_in, g_out);
-                const double tres = -(2.0 / 3.0) * sepa / dadt;
-                const double dmpc = dcom / MY_MPC;
-                const double cosmo_fact = K.four_pi_c_mpc * (1.0 + new_redz) * (dmpc * dmpc);
-                dn[ff] = nd * tres * cosmo_fact;
+                if (new_time > age_universe) break;
+                rf[ff] = (double)step;
             }
         }
+    }
+    __syncthreads();   // phase-1 rows (global memory, this workgroup's own) are complete
+    // phase 2: the per-target work for every recorded (kk, ff)
+    const int nzf = Z * F;
+    for (int p = tid; p < nzf; p += blockDim.x) {
+        const double sv = rf0[p];
+        if (sv < 0.0) {
+            dn0[p] = 0.0;
+            continue;
+        }
+        const int step = (int)sv;
+        const int kk = p / F, ff = p - (p / F) * F;
+        const double gmt = gmt_time[row0 + kk];
+        const double nd = nden[row0 + kk];
+        const double dt = s_dt[step + 1];
+        const double time_right = s_tevo[step + 1] + gmt + s_agek[kk];
+        const double time_left = time_right - dt;
+        const int idx_left = bracket_increasing(n_interp, time_left, s_gage);
+        const double redz_left = interp_at_index(idx_left, time_left, s_gage, s_gz);
+        const int idx_right = bracket_increasing(n_interp, time_right, s_gage);
+        double redz_right = interp_at_index(idx_right, time_right, s_gage, s_gz);
+        if (redz_right < 0.0) redz_right = 0.0;
+        const double fl = s_frst[step] / (1.0 + redz_left);
+        const double fr = s_frst[step + 1] / (1.0 + redz_right);
+        const double ft = fobs[ff];
+        const double new_time = interp_between(ft, fl, fr, time_left, time_right);
+        const int inew = ww_increasing(idx_left, n_interp, new_time, s_gage);
+        const double new_redz = interp_at_index(inew, new_time, s_gage, s_gz);
+        const double dcom = interp_at_index(inew, new_time, s_gage, s_gdc);
+        rf0[p] = new_redz;
+        const double frst = ft * (1.0 + new_redz);
+        const double sepa = kepler_sepa_from_freq(K, mt, frst);
+        const double dadt = hard_func_2pwl_gw(K, mt, mr, sepa, norm, rchar, g_in, g_out);
+        const double tres = -(2.0 / 3.0) * sepa / dadt;
+        const double dmpc = dcom / MY_MPC;
+        const double cosmo_fact = K.four_pi_c_mpc * (1.0 + new_redz) * (dmpc * dmpc);
+        dn0[p] = nd * tres * cosmo_fact;
     }
 }
 
@@ -409,9 +465,12 @@ extern "C" int hd_dbn_2pwl(const double* fobs_orb, int F, int fobs_sorted, doubl
         return set_error(HD_ERR_ARG, "hd_dbn_2pwl: bad sizes F=%d M=%d Q=%d Z=%d n_interp=%d steps=%d", F, M, Q, Z,
                          n_interp, num_steps);
     if (num_steps > DBN_MAX_STEPS) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: num_steps=%d > %d", num_steps, DBN_MAX_STEPS);
-    if (n_interp > DBN_MAX_TAB) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: n_interp=%d > %d", n_interp, DBN_MAX_TAB);
     if ((int64_t)M * Q > 0x7fffffff) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: M*Q too large");
-    hipLaunchKernelGGL(k_dbn_2pwl, dim3(M * Q), dim3(128), 0, (hipStream_t)stream, evo_consts(), fobs_orb, F,
+    if ((int64_t)Z * F > 0x7fffffff) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: Z*F too large");
+    const size_t lds = ((size_t)5 * (num_steps + 1) + 3 * (size_t)n_interp + (size_t)Z) * sizeof(double);
+    if (lds > 65536)
+        return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: step/cosmology tables need %zu B of LDS (> 64 KiB)", lds);
+    hipLaunchKernelGGL(k_dbn_2pwl, dim3(M * Q), dim3(DBN_THREADS), lds, (hipStream_t)stream, evo_consts(), fobs_orb, F,
                        fobs_sorted, sepa_init, num_steps, hard_norm, rchar, gamma_inner, gamma_outer, nden, mtot, M,
                        mrat, Q, redz, Z, gmt_time, grid_z, grid_dcom, grid_age, n_interp, redz_final, diff_num);
     return check_launch("k_dbn_2pwl");

@@ -45,8 +45,8 @@ struct RealizeArgs {
     // partials
     double* p_total;      // (F, nchunks, R)
     double* p_ptot;       // (npar, F, nchunks, R)
-    uint64_t* p_key;      // (F, nchunks, R, L)
-    int32_t* p_bin;       // (F, nchunks, R, L)
+    uint64_t* p_key;      // (F, nchunks, L, R): realization fastest
+    int32_t* p_bin;       // (F, nchunks, L, R)
 };
 
 // ascending image of a double, so that larger h -> smaller key when inverted
@@ -182,8 +182,9 @@ __global__ void __launch_bounds__(512) k_realize_chunk(RealizeArgs A) {
 #pragma unroll
                 for (int j = 0; j < LMAX; j++)
                     if (j < L) {
-                        A.p_key[pr * L + j] = lk[j];
-                        A.p_bin[pr * L + j] = lb[j];
+                        const int64_t o = ((pr / R) * L + j) * R + (pr % R);
+                        A.p_key[o] = lk[j];
+                        A.p_bin[o] = lb[j];
                     }
             }
         }
@@ -215,8 +216,9 @@ __global__ void __launch_bounds__(256) k_realize_merge(RealizeArgs A, double* __
         for (int k = 0; k < A.npar; k++) ptot[k] += A.p_ptot[(int64_t)k * F * NC * R + pr];
         if (use_list) {
             for (int j = 0; j < L; j++) {
-                uint64_t k = A.p_key[pr * L + j];
-                int32_t b = A.p_bin[pr * L + j];
+                const int64_t o = (((int64_t)f * NC + c) * L + j) * R + r;
+                uint64_t k = A.p_key[o];
+                int32_t b = A.p_bin[o];
                 if (k == KEY_NONE) break;
                 uint64_t wk = KEY_NONE;
                 int32_t wb = 0x7fffffff;

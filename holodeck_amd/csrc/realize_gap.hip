@@ -13,18 +13,37 @@
 //     assigned to the bin whose interval holds it.  This is exact (disjoint
 //     intervals of a Poisson process have independent Poisson counts) and
 //     costs O(sum lambda) per realization instead of O(#bins);
-//   * lambda >= tau ("dense"): one draw per bin: inversion below 10, PTRS
-//     (Hoermann 1993) at and above 10, Normal(lambda, sqrt(lambda)) above the
-//     normal threshold, as the reference (cyutils.pyx:1328-1332).
+//   * lambda >= tau ("dense"): one draw per bin and realization.
 //
-// Two kernels:
+// Every bin is drawn R times (once per realization), so instead of a
+// rejection sampler per draw the workgroup builds the bin's CDF once, in LDS,
+// over a window [klo, khi] around lambda, and every realization inverts its
+// uniform by a branch-free binary search.  With b = 56 ln 2 the window is
+//   klo = lambda - sqrt(2 b lambda)              (Chernoff lower tail:
+//         P(X <= lambda - x) <= exp(-x^2 / 2 lambda))
+//   khi = lambda + b/3 + sqrt(b^2/9 + 2 b lambda) (Bernstein upper tail:
+//         P(X >= lambda + x) <= exp(-x^2 / (2 (lambda + x/3))))
+// so each dropped tail holds less than 2^-56, below the 2^-53 resolution of the
+// uniforms; a uniform above the table's last value (CDF rounding, ~n ulp)
+// returns khi.  Bins whose window does not fit the LDS table, and lambda above
+// the normal threshold, are drawn by k_gz_big afterwards (PTRS, Hoermann 1993;
+// Normal(lambda, sqrt(lambda)) unfloored as cyutils.pyx:1328-1332), which
+// continues the same partials.  The sparse tile total N uses the same CDF
+// machinery (as a pseudo-record appended to the tile; its lambda is below
+// 512 tau, so its window always fits).
+//
+// Three kernels:
 //   k_gz_prep  -- per (tile of 512 bins, frequency): classify, compact into
-//                 per-tile dense records (sampler constants precomputed once
-//                 for all realizations) and sparse records (cumulative lambda);
-//   k_gz_draw  -- per (chunk of tiles, frequency): every realization (one per
-//                 thread, one Philox block per dense draw attempt); dense
-//                 records are read with wave-uniform loads, sparse points are
-//                 placed by binary search on the tile's cumulative axis.
+//                 per-tile dense records (lambda, h, key, CDF window and its
+//                 offset in the tile's concatenated tables) and sparse records
+//                 (cumulative lambda);
+//   k_gz_draw  -- per (chunk of tiles, frequency) workgroup, one realization
+//                 per thread: tables of consecutive records are built in LDS
+//                 in batches that fit, one wave per record; then every
+//                 realization draws from them;
+//   k_gz_big   -- per (chunk, frequency), only where the chunk has records
+//                 without a table: reloads the partials, draws those records,
+//                 writes the partials back.
 // Per-(frequency, chunk, realization) partials (FP64 totals and the L best
 // candidates) are merged by realize.hip's merge kernel.
 #include "common.hpp"
@@ -34,14 +53,21 @@
 namespace hd {
 
 constexpr int GZ_TILE = 512;          // bins per tile (prep workgroup size)
+constexpr int GZ_REC = GZ_TILE + 1;   // record slots per tile (+1 sparse pseudo-record)
 constexpr int GZ_TILES_PER_CHUNK = 16;
-constexpr int GZ_THREADS = 256;       // draw workgroup: one realization per thread
+constexpr int GZ_MAXBS = 512;         // draw workgroup: one realization per thread
+constexpr int GZ_XCDS = 8;            // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr uint64_t GZ_KEY_NONE = ~0ull;
+constexpr double GZ_TAILB = 38.816242111356935;   // 56 ln 2: each dropped tail < 2^-56
 
-// dense record: 8 doubles
+// dense record (and the sparse pseudo-record at index nd)
 struct DRec {
-    double lam, h, c0, c1, c2, c3, c4;
-    uint64_t word;   // bin << 32 | rank
+    double lam, h;
+    uint64_t word;    // bin << 32 | rank
+    int32_t klo;      // first count of the CDF window
+    int32_t n;        // window length (0: no table, use the fallback sampler)
+    int32_t off;      // offset of the window in the tile's concatenated tables
+    int32_t pad;
 };
 
 struct GapArgs {
@@ -51,22 +77,22 @@ struct GapArgs {
     const int64_t* order;   // (B) (SORTED) or null
     const double* par;      // (npar,B,F) or null
     int64_t B;
-    int F, R, L, flags, npar, ntiles, nchunks, mode;
+    int F, R, L, flags, npar, ntiles, nchunks, mode, cap;
     double thresh, tau;
     uint64_t seed;
-    // tile records, region (f, tile) holds up to GZ_TILE entries
-    DRec* drec;             // (F, ntiles, GZ_TILE)
+    DRec* drec;             // (F, ntiles, GZ_REC)
+    int32_t* rec_end;       // (F, ntiles, GZ_REC) off + n, nondecreasing within a tile
+    int32_t* cnt;           // (F, ntiles, 3): dense, sparse, no-table counts
     double* s_cum;          // (F, ntiles, GZ_TILE) inclusive cumulative lambda
     double* s_h;            // (F, ntiles, GZ_TILE)
     uint64_t* s_word;       // (F, ntiles, GZ_TILE)
     double* d_par;          // (npar, F, ntiles, GZ_TILE)
     double* sp_par;         // (npar, F, ntiles, GZ_TILE)
-    int32_t* cnt;           // (F, ntiles, 2): dense, sparse counts
     // partials
     double* p_total;        // (F, nchunks, R)
     double* p_ptot;         // (npar, F, nchunks, R)
-    uint64_t* p_key;        // (F, nchunks, R, L)
-    int32_t* p_bin;         // (F, nchunks, R, L)
+    uint64_t* p_key;        // (F, nchunks, L, R): realization fastest
+    int32_t* p_bin;         // (F, nchunks, L, R)
 };
 
 __device__ __forceinline__ uint64_t gz_hkey(double h) {
@@ -75,13 +101,56 @@ __device__ __forceinline__ uint64_t gz_hkey(double h) {
     return ~u;
 }
 
+// CDF window of Poisson(lam): [klo, klo + n); n = 0 if it exceeds cap entries
+__device__ __forceinline__ void gz_window(double lam, int cap, int& klo, int& n) {
+    const double b = GZ_TAILB;
+    const double lo = fmax(floor(lam - sqrt(2.0 * b * lam)), 0.0);
+    const double hi = ceil(lam + b / 3.0 + sqrt(b * b / 9.0 + 2.0 * b * lam));
+    const double w = hi - lo + 1.0;
+    if (!(w <= (double)cap)) {
+        klo = 0;
+        n = 0;
+    } else {
+        klo = (int)lo;
+        n = (int)w;
+    }
+}
+
+// log(k!) for k = 0..15 (exact to double rounding)
+__constant__ double gz_logfact[16] = {
+    0.0, 0.0, 0.69314718055994530942, 1.79175946922805500081, 3.17805383034794561964, 4.78749174278204599424,
+    6.57925121201010099506, 8.52516136106541430017, 10.60460290274525022842, 12.80182748008146961121,
+    15.10441257307551529523, 17.50230784587388583929, 19.98721449566188614952, 22.55216385312342288557,
+    25.19122118273868150009, 27.89927138384089156609};
+
+// log(k!) for integer k >= 0: table below 16, Stirling series in x = k+1 above
+// (truncation error of the omitted x^-9 term < 1e-14 absolute at x = 17).
+__device__ __forceinline__ double gz_logfact_k(double k) {
+    if (k < 16.0) return gz_logfact[(int)k];
+    const double x = k + 1.0;
+    const double r = 1.0 / x, r2 = r * r;
+    const double series = r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0 - r2 * (1.0 / 1680.0))));
+    return (x - 0.5) * log(x) - x + 0.91893853320467274178 + series;
+}
+
+__device__ __forceinline__ double gz_pmf(double lam, double loglam, double k) {
+    return exp(k * loglam - lam - gz_logfact_k(k));
+}
+
 // ---------------------------------------------------------------------------
 // prep: classify and compact one tile of one frequency
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
-    __shared__ int s_wd[GZ_TILE / 64], s_ws[GZ_TILE / 64];
+    __shared__ int s_wd[GZ_TILE / 64], s_ws[GZ_TILE / 64], s_wn[GZ_TILE / 64], s_wb[GZ_TILE / 64];
     __shared__ double s_wsum[GZ_TILE / 64];
-    const int f = blockIdx.y, tile = blockIdx.x;
+    // XCD-aware order: workgroup w runs on XCD w % 8; the workgroups of one XCD
+    // take one tile for every frequency in turn, so the tile's rows of number
+    // and h2fdf (F doubles each) are fetched once into that XCD's L2.
+    const int w = blockIdx.x;
+    const int k8 = w / GZ_XCDS;
+    const int f = k8 % A.F;
+    const int tile = (k8 / A.F) * GZ_XCDS + (w % GZ_XCDS);
+    if (tile >= A.ntiles) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int F = A.F;
     const int64_t b = (int64_t)tile * GZ_TILE + tid;
@@ -98,28 +167,48 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
     }
     const bool dense = in && (lam >= A.tau);
     const bool sparse = in && !dense;
-    const uint64_t md = __ballot(dense), ms = __ballot(sparse);
+    int klo = 0, nwin = 0;
+    if (dense && lam <= A.thresh) gz_window(lam, A.cap, klo, nwin);
+    const uint64_t md = __ballot(dense), ms = __ballot(sparse), mb = __ballot(dense && nwin == 0);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     double x = sparse ? lam : 0.0;
+    int xn = nwin;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        double y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
+        const double y = __shfl_up(x, off, 64);
+        const int yn = __shfl_up(xn, off, 64);
+        if (lane >= off) {
+            x += y;
+            xn += yn;
+        }
     }
-    if (lane == 63) s_wsum[wave] = x;
+    if (lane == 63) {
+        s_wsum[wave] = x;
+        s_wn[wave] = xn;
+    }
     if (lane == 0) {
         s_wd[wave] = __popcll(md);
         s_ws[wave] = __popcll(ms);
+        s_wb[wave] = __popcll(mb);
     }
     __syncthreads();
-    int od = 0, os = 0;
+    int od = 0, os = 0, on = 0, nd = 0, ns = 0, ntot = 0, nbig = 0;
     double osum = 0.0;
-    for (int w = 0; w < wave; w++) {
-        od += s_wd[w];
-        os += s_ws[w];
-        osum += s_wsum[w];
+    for (int w = 0; w < GZ_TILE / 64; w++) {
+        if (w < wave) {
+            od += s_wd[w];
+            os += s_ws[w];
+            on += s_wn[w];
+            osum += s_wsum[w];
+        }
+        nd += s_wd[w];
+        ns += s_ws[w];
+        ntot += s_wn[w];
+        nbig += s_wb[w];
     }
-    const int64_t reg = ((int64_t)f * A.ntiles + tile) * GZ_TILE;
+    const int64_t ft = (int64_t)f * A.ntiles + tile;
+    const int64_t reg = ft * GZ_TILE;
+    const int64_t rreg = ft * GZ_REC;
     const uint32_t rk = (A.mode == HD_SEL_SORTED && in) ? (uint32_t)A.rank[b] : 0u;
     const uint64_t word = ((uint64_t)(uint32_t)b << 32) | rk;
     if (dense) {
@@ -127,44 +216,40 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
         DRec r;
         r.lam = lam;
         r.h = h;
-        r.c0 = r.c1 = r.c2 = r.c3 = r.c4 = 0.0;
-        if (lam > A.thresh) {
-            r.c0 = sqrt(lam);
-        } else if (lam < 10.0) {
-            r.c0 = -expm1(-lam);
-            r.c1 = exp(-lam);
-        } else {
-            const double slam = sqrt(lam);
-            const double bb = 0.931 + 2.53 * slam;
-            const double aa = -0.059 + 0.02483 * bb;
-            const double invalpha = 1.1239 + 1.1328 / (bb - 3.4);
-            r.c0 = bb;
-            r.c1 = aa;
-            r.c2 = log(invalpha);
-            r.c3 = 0.9277 - 3.6224 / (bb - 2);
-            r.c4 = log(lam);
-        }
         r.word = word;
-        A.drec[reg + i] = r;
+        r.klo = klo;
+        r.n = nwin;
+        r.off = on + xn - nwin;   // exclusive prefix of window lengths
+        r.pad = 0;
+        A.drec[rreg + i] = r;
+        A.rec_end[rreg + i] = r.off + nwin;
         for (int k = 0; k < A.npar; k++)
             A.d_par[(int64_t)k * F * A.ntiles * GZ_TILE + reg + i] = A.par[((int64_t)k * A.B + b) * F + f];
     }
     if (sparse) {
         const int i = os + __popcll(ms & lt);
-        A.s_cum[reg + i] = osum + x;
+        const double cum = osum + x;
+        A.s_cum[reg + i] = cum;
         A.s_h[reg + i] = h;
         A.s_word[reg + i] = word;
         for (int k = 0; k < A.npar; k++)
             A.sp_par[(int64_t)k * F * A.ntiles * GZ_TILE + reg + i] = A.par[((int64_t)k * A.B + b) * F + f];
+        if (i == ns - 1) {   // the last sparse bin writes the pseudo-record for the tile's point count
+            DRec r;
+            r.lam = cum;
+            r.h = 0.0;
+            r.word = 0;
+            gz_window(cum, A.cap, r.klo, r.n);
+            r.off = ntot;
+            r.pad = 0;
+            A.drec[rreg + nd] = r;
+            A.rec_end[rreg + nd] = ntot + r.n;
+        }
     }
     if (tid == 0) {
-        int nd = 0, ns = 0;
-        for (int w = 0; w < GZ_TILE / 64; w++) {
-            nd += s_wd[w];
-            ns += s_ws[w];
-        }
-        A.cnt[(((int64_t)f * A.ntiles) + tile) * 2] = nd;
-        A.cnt[(((int64_t)f * A.ntiles) + tile) * 2 + 1] = ns;
+        A.cnt[ft * 3] = nd;
+        A.cnt[ft * 3 + 1] = ns;
+        A.cnt[ft * 3 + 2] = nbig;
     }
 }
 
@@ -203,6 +288,14 @@ struct LoudList<HD_SEL_SORTED, LM> {
     }
     __device__ __forceinline__ uint64_t key_at(int j) const {
         return k[j] == 0xffffffffu ? GZ_KEY_NONE : (uint64_t)k[j];
+    }
+    __device__ __forceinline__ void load(const uint64_t* pk, const int32_t*, int L, int64_t stride) {
+#pragma unroll
+        for (int i = 0; i < LM; i++) {
+            const uint64_t key = (i < L) ? pk[i * stride] : GZ_KEY_NONE;
+            k[i] = (key == GZ_KEY_NONE) ? 0xffffffffu : (uint32_t)key;
+            if (i == L - 1) wk = k[i];
+        }
     }
     __device__ __forceinline__ int32_t bin_at(int j, const int64_t* order) const {
         return k[j] == 0xffffffffu ? 0x7fffffff : (int32_t)order[k[j]];
@@ -250,50 +343,19 @@ struct LoudList {
             }
     }
     __device__ __forceinline__ uint64_t key_at(int j) const { return k[j]; }
+    __device__ __forceinline__ void load(const uint64_t* pk, const int32_t* pb, int L, int64_t stride) {
+#pragma unroll
+        for (int i = 0; i < LM; i++) {
+            k[i] = (i < L) ? pk[i * stride] : GZ_KEY_NONE;
+            b[i] = (i < L) ? pb[i * stride] : 0x7fffffff;
+            if (i == L - 1) {
+                wk = k[i];
+                wb = b[i];
+            }
+        }
+    }
     __device__ __forceinline__ int32_t bin_at(int j, const int64_t*) const { return b[j]; }
 };
-
-// inversion of the upper tail for 0 < lambda < 10 with a 53-bit uniform
-__device__ __forceinline__ double gz_inversion(double u, double lam, double q1, double elam) {
-    int k = 0;
-    double S = q1, p = elam;
-    while (u < S && k < 200) {
-        k++;
-        p = p * lam / k;
-        S = S - p;
-    }
-    return (double)k;
-}
-
-// PTRS acceptance with a uniform pair (U in (-0.5,0.5), V); returns k or -1 (reject)
-__device__ __forceinline__ double gz_ptrs_try(double U, double V, const DRec& r) {
-    const double us = 0.5 - fabs(U);
-    const double k = floor((2 * r.c1 / us + r.c0) * U + r.lam + 0.43);
-    if ((us >= 0.07) && (V <= r.c3)) return k;
-    if ((k < 0) || ((us < 0.013) && (V > us))) return -1.0;
-    const bool acc = (log(V) + r.c2 - log(r.c1 / (us * us) + r.c0)) <= (-r.lam + k * r.c4 - lgamma(k + 1.0));
-    return acc ? k : -1.0;
-}
-
-// Count of one dense bin for realization r: Philox counter (bin, f, r,
-// tag 1 | attempt).  Inversion uses words (x,y); a PTRS attempt uses (x,y) for
-// U and (z,w) for V; the Normal branch uses both pairs for Box-Muller.
-__device__ __forceinline__ double gz_dense_draw(const DRec& rec, uint32_t f, uint32_t r, uint64_t seed, double thresh) {
-    const uint32_t bin = (uint32_t)(rec.word >> 32);
-    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-    const uint32_t TAG = 1u << 16;
-    U4 a = philox4x32_10(U4{bin, f, r, TAG}, k0, k1);
-    if (rec.lam < 10.0) return gz_inversion(u53(a.x, a.y), rec.lam, rec.c0, rec.c1);
-    if (rec.lam > thresh)
-        return rec.lam + rec.c0 * (sqrt(-2.0 * log(u53(a.x, a.y))) * cos(2.0 * M_PI * u53(a.z, a.w)));
-    double n = gz_ptrs_try(u53(a.x, a.y) - 0.5, u53(a.z, a.w), rec);
-#pragma unroll 1
-    for (uint32_t att = 1; n < 0 && att < 4000u; att++) {
-        a = philox4x32_10(U4{bin, f, r, TAG | att}, k0, k1);
-        n = gz_ptrs_try(u53(a.x, a.y) - 0.5, u53(a.z, a.w), rec);
-    }
-    return (n < 0) ? floor(rec.lam) : n;   // the bound is unreachable in practice
-}
 
 template <int MODE, int LM, int NP>
 struct Acc {
@@ -307,11 +369,13 @@ struct Acc {
         for (int k = 0; k < (NP > 0 ? NP : 1); k++) pt[k] = 0.0;
     }
     // n instances of a bin with strain h (n as drawn; the sorted walk ignores n < 1)
-    __device__ __forceinline__ void add(double n, double h, uint64_t word, const double* p, int64_t pstride, int L) {
+    __device__ __forceinline__ void add(double n, double h, uint64_t word, const double* p, int64_t pstride, int npar,
+                                        int L) {
         if (MODE == HD_SEL_SORTED ? !(n >= 1.0) : (n == 0.0)) return;
         tot += n * h;
 #pragma unroll
-        for (int k = 0; k < NP; k++) pt[k] += n * h * p[k * pstride];
+        for (int k = 0; k < NP; k++)
+            if (k < npar) pt[k] += n * h * p[k * pstride];
         if (MODE != HD_SEL_NONE && n > 0 && list.admits(word, h)) {
             const int ninst = (int)fmin(ceil(n), (double)L);
             for (int i = 0; i < ninst && list.admits(word, h); i++) list.insert(word, h, L);
@@ -319,17 +383,93 @@ struct Acc {
     }
 };
 
+// Build the CDF window of one record into tab[0, n) with one wave: each lane
+// takes a contiguous segment, starts from the exact pmf and recurses
+// p(k+1) = p(k) lam / (k+1); segment sums are joined by a wave scan.
+__device__ __noinline__ void gz_build(double* tab, double lam, int klo, int n, int lane) {
+    const int s = (n + 63) >> 6;
+    const int j0 = lane * s;
+    const int m = min(max(n - j0, 0), s);
+    const double loglam = log(lam);
+    double local = 0.0;
+    if (m > 0) {
+        double k = (double)(klo + j0);
+        double p = gz_pmf(lam, loglam, k);
+        for (int j = 0; j < m; j++) {
+            local += p;
+            tab[j0 + j] = local;
+            k += 1.0;
+            p = p * lam / k;
+        }
+    }
+    double x = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    const double carry = x - local;
+    if (m > 0 && lane > 0)
+        for (int j = 0; j < m; j++) tab[j0 + j] += carry;
+}
+
+// Inversion: the smallest k in the window with CDF(k) > u; a uniform above the
+// table's last value (rounding of the CDF near 1) returns the window's end.
+__device__ __forceinline__ double gz_table_draw(const double* tab, int klo, int n, double u) {
+    int base = 0, len = n;
+    while (len > 1) {   // invariant: the answer is in [base, base + len]
+        const int half = len >> 1;
+        base = (tab[base + half] <= u) ? base + half : base;
+        len -= half;
+    }
+    return (double)(klo + min(base + (tab[base] <= u ? 1 : 0), n - 1));
+}
+
+// Counts without a table: Normal above the threshold, inversion below 10,
+// PTRS otherwise; stream (a, f, r, tag).
+__device__ __forceinline__ double gz_fallback(uint32_t a, uint32_t f, uint32_t r, uint32_t tag, uint64_t seed,
+                                              double lam, double thresh) {
+    Stream st;
+    st.init(a, f, r, tag, seed);
+    if (lam > thresh) return normal_count(st, lam);
+    if (lam < 10.0) return poisson_inversion(st.next(), lam, -expm1(-lam), exp(-lam));
+    return poisson_ptrs(st, lam);
+}
+
 template <int MODE, int LM, int NP>
-__global__ void __launch_bounds__(GZ_THREADS) k_gz_draw(GapArgs A) {
+__device__ __forceinline__ void gz_store(const GapArgs& A, const Acc<MODE, LM, NP>& acc, int64_t pr) {
+    const int F = A.F, R = A.R, L = A.L;
+    A.p_total[pr] = acc.tot;
+#pragma unroll
+    for (int k = 0; k < NP; k++)
+        if (k < A.npar) A.p_ptot[(int64_t)k * F * A.nchunks * R + pr] = acc.pt[k];
+    if (MODE != HD_SEL_NONE) {
+#pragma unroll
+        for (int j = 0; j < LM; j++) {
+            if (j >= L) continue;
+            const int64_t o = ((pr / R) * L + j) * R + (pr % R);
+            A.p_key[o] = acc.list.key_at(j);
+            A.p_bin[o] = acc.list.bin_at(j, A.order);
+        }
+    }
+}
+
+template <int MODE, int LM, int NP>
+__global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
+    extern __shared__ double gz_lds[];
+    double* tab = gz_lds;              // A.cap doubles
+    double* scum = gz_lds + A.cap;     // GZ_TILE doubles
     const int f = blockIdx.y;
     const int chunk = blockIdx.x;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bs = blockDim.x, nw = bs >> 6;
     const int F = A.F, R = A.R, L = A.L;
     const int t0 = chunk * GZ_TILES_PER_CHUNK;
     const int t1 = min(t0 + GZ_TILES_PER_CHUNK, A.ntiles);
     const int64_t pstride = (int64_t)F * A.ntiles * GZ_TILE;
+    const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
 
-    for (int rbase = 0; rbase < R; rbase += GZ_THREADS) {
+    for (int rbase = 0; rbase < R; rbase += bs) {
         const int r = rbase + tid;
         const bool active = r < R;
         Acc<MODE, LM, NP> acc;
@@ -337,50 +477,107 @@ __global__ void __launch_bounds__(GZ_THREADS) k_gz_draw(GapArgs A) {
         for (int tile = t0; tile < t1; tile++) {
             const int64_t ft = (int64_t)f * A.ntiles + tile;
             const int64_t reg = ft * GZ_TILE;
-            const int nd = A.cnt[ft * 2];
-            const int ns = A.cnt[ft * 2 + 1];
-            // dense bins: wave-uniform loop, record in scalar registers
-            for (int c = 0; c < nd; c++) {
-                const DRec rec = A.drec[reg + c];
-                const double n = gz_dense_draw(rec, (uint32_t)f, (uint32_t)r, A.seed, A.thresh);
-                if (active) acc.add(n, rec.h, rec.word, A.d_par + reg + c, pstride, L);
-            }
-            // sparse bins: Poisson process on the tile's cumulative-lambda axis
-            if (ns > 0 && active) {
-                const double* cum = A.s_cum + reg;
-                const double Lam = cum[ns - 1];
-                Stream st;
-                st.init((uint32_t)tile, (uint32_t)f, (uint32_t)r, 2u, A.seed);
-                double npts;
-                if (Lam < 10.0) npts = poisson_inversion(st.next(), Lam, -expm1(-Lam), exp(-Lam));
-                else npts = poisson_ptrs(st, Lam);
-                const int np = (int)npts;
-                for (int i = 0; i < np; i++) {
-                    const double x = st.next() * Lam;
-                    int lo = 0, hi = ns - 1;
-                    while (lo < hi) {   // first c with cum[c] > x
-                        const int mid = (lo + hi) >> 1;
-                        if (cum[mid] > x) hi = mid;
-                        else lo = mid + 1;
+            const int64_t rreg = ft * GZ_REC;
+            const int nd = A.cnt[ft * 3];
+            const int ns = A.cnt[ft * 3 + 1];
+            const int nrec = nd + (ns > 0 ? 1 : 0);
+            int c = 0;
+            while (c < nrec) {
+                // batch [c, c1): consecutive records whose windows fit the table
+                const int base = (c == 0) ? 0 : A.rec_end[rreg + c - 1];
+                const int limit = base + A.cap;
+                int c1 = c;
+                for (;;) {
+                    const int idx = c1 + lane;
+                    const int e = (idx < nrec) ? A.rec_end[rreg + idx] : 0x7fffffff;
+                    const int adv = __popcll(__ballot(e <= limit));
+                    c1 += adv;
+                    if (adv < 64) break;
+                }
+                c1 = max(c1, c + 1);   // every window fits alone (n <= cap); keeps progress unconditional
+                __syncthreads();   // the previous batch's draws are done with tab / scum
+                for (int i = c + wave; i < c1; i += nw) {
+                    const DRec rec = A.drec[rreg + i];
+                    if (rec.n > 0) gz_build(tab + (rec.off - base), rec.lam, rec.klo, rec.n, lane);
+                }
+                if (c1 == nrec && ns > 0)
+                    for (int j = tid; j < ns; j += bs) scum[j] = A.s_cum[reg + j];
+                __syncthreads();
+                if (active) {
+                    for (int i = c; i < c1; i++) {
+                        const DRec rec = A.drec[rreg + i];
+                        if (i < nd) {
+                            if (rec.n == 0) continue;   // no table: drawn by k_gz_big
+                            const uint32_t bin = (uint32_t)(rec.word >> 32);
+                            const U4 a = philox4x32_10(U4{bin, (uint32_t)f, (uint32_t)r, 1u << 16}, k0, k1);
+                            const double n = gz_table_draw(tab + (rec.off - base), rec.klo, rec.n, u53(a.x, a.y));
+                            acc.add(n, rec.h, rec.word, A.d_par + reg + i, pstride, A.npar, L);
+                        } else {
+                            // sparse bins: Poisson process on the tile's cumulative-lambda axis; the
+                            // point count from stream tag 4 (its window always fits: lambda < 512),
+                            // the positions from stream tag 2
+                            const double Lam = scum[ns - 1];
+                            const U4 a = philox4x32_10(U4{(uint32_t)tile, (uint32_t)f, (uint32_t)r, 4u << 16}, k0, k1);
+                            const int np = (int)gz_table_draw(tab + (rec.off - base), rec.klo, rec.n, u53(a.x, a.y));
+                            Stream st;
+                            st.init((uint32_t)tile, (uint32_t)f, (uint32_t)r, 2u, A.seed);
+                            for (int q = 0; q < np; q++) {
+                                const double xq = st.next() * Lam;
+                                int lo = 0, hi = ns - 1;
+                                while (lo < hi) {   // first j with scum[j] > xq
+                                    const int mid = (lo + hi) >> 1;
+                                    if (scum[mid] > xq) hi = mid;
+                                    else lo = mid + 1;
+                                }
+                                acc.add(1.0, A.s_h[reg + lo], A.s_word[reg + lo], A.sp_par + reg + lo, pstride, A.npar,
+                                        L);
+                            }
+                        }
                     }
-                    acc.add(1.0, A.s_h[reg + lo], A.s_word[reg + lo], A.sp_par + reg + lo, pstride, L);
                 }
+                c = c1;
             }
         }
-        if (active) {
-            const int64_t pr = ((int64_t)f * A.nchunks + chunk) * R + r;
-            A.p_total[pr] = acc.tot;
+        if (active) gz_store<MODE, LM, NP>(A, acc, ((int64_t)f * A.nchunks + chunk) * R + r);
+    }
+}
+
+// Records without a table (lambda above the threshold or a window larger than
+// the LDS table): reload the chunk's partials, draw, store them back.
+template <int MODE, int LM, int NP>
+__global__ void __launch_bounds__(256) k_gz_big(GapArgs A) {
+    const int f = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const int F = A.F, R = A.R, L = A.L;
+    const int t0 = chunk * GZ_TILES_PER_CHUNK;
+    const int t1 = min(t0 + GZ_TILES_PER_CHUNK, A.ntiles);
+    const int64_t pstride = (int64_t)F * A.ntiles * GZ_TILE;
+    int nbig = 0;
+    for (int tile = t0; tile < t1; tile++) nbig += A.cnt[((int64_t)f * A.ntiles + tile) * 3 + 2];
+    if (nbig == 0) return;
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+        const int64_t pr = ((int64_t)f * A.nchunks + chunk) * R + r;
+        Acc<MODE, LM, NP> acc;
+        acc.tot = A.p_total[pr];
 #pragma unroll
-            for (int k = 0; k < NP; k++) A.p_ptot[(int64_t)k * F * A.nchunks * R + pr] = acc.pt[k];
-            if (MODE != HD_SEL_NONE) {
-#pragma unroll
-                for (int j = 0; j < LM; j++) {
-                    if (j >= L) continue;
-                    A.p_key[pr * L + j] = acc.list.key_at(j);
-                    A.p_bin[pr * L + j] = acc.list.bin_at(j, A.order);
-                }
+        for (int k = 0; k < NP; k++) acc.pt[k] = (k < A.npar) ? A.p_ptot[(int64_t)k * F * A.nchunks * R + pr] : 0.0;
+        if (MODE != HD_SEL_NONE) {
+            const int64_t o = ((pr / R) * L) * R + (pr % R);
+            acc.list.load(A.p_key + o, A.p_bin + o, L, R);
+        }
+        for (int tile = t0; tile < t1; tile++) {
+            const int64_t ft = (int64_t)f * A.ntiles + tile;
+            if (A.cnt[ft * 3 + 2] == 0) continue;
+            const int nd = A.cnt[ft * 3];
+            for (int i = 0; i < nd; i++) {
+                const DRec rec = A.drec[ft * GZ_REC + i];
+                if (rec.n != 0) continue;
+                const double n = gz_fallback((uint32_t)(rec.word >> 32), (uint32_t)f, (uint32_t)r, 3u, A.seed, rec.lam,
+                                             A.thresh);
+                acc.add(n, rec.h, rec.word, A.d_par + ft * GZ_TILE + i, pstride, A.npar, L);
             }
         }
+        gz_store<MODE, LM, NP>(A, acc, pr);
     }
 }
 
@@ -392,26 +589,45 @@ int rank_from_order_launch(const int64_t* order, int64_t B, int32_t* rank, hipSt
 
 static size_t gz_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
+static int gz_block(int R) { return (int)min((int64_t)GZ_MAXBS, (int64_t)cdiv(R, 64) * 64); }
+// LDS table entries per workgroup: 16 per thread, at least 2048 (> any sparse
+// tile window), at most 7680 so that table + sparse axis stay within 64 KiB
+static int gz_cap(int bs) { return max(2048, min(7680, bs * 16)); }
+
 static size_t gz_ws(int64_t B, int F, int R, int L, int npar) {
     const int64_t nt = cdiv(B, GZ_TILE);
     const int64_t nch = cdiv(nt, GZ_TILES_PER_CHUNK);
     const size_t slots = (size_t)F * nt * GZ_TILE;
+    const size_t rslots = (size_t)F * nt * GZ_REC;
     const size_t pr = (size_t)F * nch * R;
     size_t s = gz_align((size_t)B * 4);                    // rank
-    s += gz_align(slots * sizeof(DRec));                   // dense records
+    s += gz_align(rslots * sizeof(DRec));                  // dense records
+    s += gz_align(rslots * 4);                             // record ends
+    s += gz_align((size_t)F * nt * 3 * 4);                 // counts
     s += 3 * gz_align(slots * 8);                          // sparse cum, h, word
     s += 2 * gz_align(slots * 8 * (size_t)npar);           // params
-    s += gz_align((size_t)F * nt * 2 * 4);                 // counts
     s += gz_align(pr * 8) + gz_align(pr * 8 * (size_t)npar) + gz_align(pr * 8 * (size_t)L) +
          gz_align(pr * 4 * (size_t)L);
     return s + 512;
 }
 
+template <int MODE, int LM, int NP>
+static int gz_launch(const GapArgs& A, dim3 grid, int bs, size_t lds, hipStream_t st) {
+    {
+        KTimer timer("k_gz_draw", st);
+        hipLaunchKernelGGL((k_gz_draw<MODE, LM, NP>), grid, dim3(bs), lds, st, A);
+    }
+    int rc = check_launch("k_gz_draw");
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_gz_big<MODE, LM, NP>), grid, dim3(256), 0, st, A);
+    return check_launch("k_gz_big");
+}
+
 template <int MODE, int LM>
-static void gz_launch_np(const GapArgs& A, int npar, dim3 grid, hipStream_t st) {
-    if (npar == 0) hipLaunchKernelGGL((k_gz_draw<MODE, LM, 0>), grid, dim3(GZ_THREADS), 0, st, A);
-    else if (npar <= 3) hipLaunchKernelGGL((k_gz_draw<MODE, LM, 3>), grid, dim3(GZ_THREADS), 0, st, A);
-    else hipLaunchKernelGGL((k_gz_draw<MODE, LM, 7>), grid, dim3(GZ_THREADS), 0, st, A);
+static int gz_launch_np(const GapArgs& A, int npar, dim3 grid, int bs, size_t lds, hipStream_t st) {
+    if (npar == 0) return gz_launch<MODE, LM, 0>(A, grid, bs, lds, st);
+    if (npar <= 3) return gz_launch<MODE, LM, 3>(A, grid, bs, lds, st);
+    return gz_launch<MODE, LM, 7>(A, grid, bs, lds, st);
 }
 
 }  // namespace hd
@@ -429,6 +645,7 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     if (B < 1 || F < 1 || R < 1 || !number || !h2fdf || !hc2bg) return set_error(HD_ERR_ARG, "hd_realize: bad arguments");
     if (B > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: B=%lld exceeds 2^31-1", (long long)B);
     if (F > 65535) return set_error(HD_ERR_SIZE, "hd_realize: F=%d > 65535", F);
+    if (R > 0x7fffffff / 2) return set_error(HD_ERR_SIZE, "hd_realize: R=%d too large", R);
     if (mode != HD_SEL_SORTED && mode != HD_SEL_ARGMAX && mode != HD_SEL_NONE)
         return set_error(HD_ERR_ARG, "hd_realize: unknown mode %d", mode);
     if (mode == HD_SEL_NONE) L = 0;
@@ -441,6 +658,8 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     if (!workspace || ws_bytes < need)
         return set_error(HD_ERR_SIZE, "hd_realize: workspace too small (%zu < %zu)", ws_bytes, need);
     hipStream_t st = (hipStream_t)stream;
+    KTimer timer("hd_realize", st);   // the whole launch sequence, rank to merge
+    const int bs = gz_block(R);
     GapArgs A;
     A.number = number;
     A.h2fdf = h2fdf;
@@ -453,22 +672,25 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     A.flags = flags;
     A.npar = npar;
     A.mode = mode;
+    A.cap = gz_cap(bs);
     A.thresh = normal_threshold;
-    A.tau = 1.0;
+    A.tau = fmin(1.0, normal_threshold);   // bins above the threshold are never sparse
     A.seed = seed;
     A.ntiles = (int)cdiv(B, GZ_TILE);
     A.nchunks = (int)cdiv(A.ntiles, GZ_TILES_PER_CHUNK);
     const size_t slots = (size_t)F * A.ntiles * GZ_TILE;
+    const size_t rslots = (size_t)F * A.ntiles * GZ_REC;
     const size_t pr = (size_t)F * A.nchunks * R;
     char* p = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
     int32_t* rank = (int32_t*)p;  p += gz_align((size_t)B * 4);
-    A.drec = (DRec*)p;            p += gz_align(slots * sizeof(DRec));
+    A.drec = (DRec*)p;            p += gz_align(rslots * sizeof(DRec));
+    A.rec_end = (int32_t*)p;      p += gz_align(rslots * 4);
+    A.cnt = (int32_t*)p;          p += gz_align((size_t)F * A.ntiles * 3 * 4);
     A.s_cum = (double*)p;         p += gz_align(slots * 8);
     A.s_h = (double*)p;           p += gz_align(slots * 8);
     A.s_word = (uint64_t*)p;      p += gz_align(slots * 8);
     A.d_par = (double*)p;         p += gz_align(slots * 8 * (size_t)npar);
     A.sp_par = (double*)p;        p += gz_align(slots * 8 * (size_t)npar);
-    A.cnt = (int32_t*)p;          p += gz_align((size_t)F * A.ntiles * 2 * 4);
     A.p_total = (double*)p;       p += gz_align(pr * 8);
     A.p_ptot = (double*)p;        p += gz_align(pr * 8 * (size_t)npar);
     A.p_key = (uint64_t*)p;       p += gz_align(pr * 8 * (size_t)L);
@@ -478,22 +700,24 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
         int rc = rank_from_order_launch(order, B, rank, st);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_gz_prep, dim3(A.ntiles, F), dim3(GZ_TILE), 0, st, A);
+    const int64_t prep_blocks = cdiv(A.ntiles, GZ_XCDS) * GZ_XCDS * (int64_t)F;
+    if (prep_blocks > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: too many tiles x frequencies");
+    hipLaunchKernelGGL(k_gz_prep, dim3((unsigned)prep_blocks), dim3(GZ_TILE), 0, st, A);
     int rc = check_launch("k_gz_prep");
     if (rc) return rc;
     const dim3 grid(A.nchunks, F);
+    const size_t lds = (size_t)(A.cap + GZ_TILE) * sizeof(double);
     if (mode == HD_SEL_SORTED) {
-        if (L <= 1) gz_launch_np<HD_SEL_SORTED, 1>(A, npar, grid, st);
-        else if (L <= 4) gz_launch_np<HD_SEL_SORTED, 4>(A, npar, grid, st);
-        else gz_launch_np<HD_SEL_SORTED, 16>(A, npar, grid, st);
+        if (L <= 1) rc = gz_launch_np<HD_SEL_SORTED, 1>(A, npar, grid, bs, lds, st);
+        else if (L <= 4) rc = gz_launch_np<HD_SEL_SORTED, 4>(A, npar, grid, bs, lds, st);
+        else rc = gz_launch_np<HD_SEL_SORTED, 16>(A, npar, grid, bs, lds, st);
     } else if (mode == HD_SEL_ARGMAX) {
-        if (L <= 1) gz_launch_np<HD_SEL_ARGMAX, 1>(A, npar, grid, st);
-        else if (L <= 4) gz_launch_np<HD_SEL_ARGMAX, 4>(A, npar, grid, st);
-        else gz_launch_np<HD_SEL_ARGMAX, 16>(A, npar, grid, st);
+        if (L <= 1) rc = gz_launch_np<HD_SEL_ARGMAX, 1>(A, npar, grid, bs, lds, st);
+        else if (L <= 4) rc = gz_launch_np<HD_SEL_ARGMAX, 4>(A, npar, grid, bs, lds, st);
+        else rc = gz_launch_np<HD_SEL_ARGMAX, 16>(A, npar, grid, bs, lds, st);
     } else {
-        gz_launch_np<HD_SEL_NONE, 1>(A, npar, grid, st);
+        rc = gz_launch_np<HD_SEL_NONE, 1>(A, npar, grid, bs, lds, st);
     }
-    rc = check_launch("k_gz_draw");
     if (rc) return rc;
     return realize_merge_launch(h2fdf, par, B, F, R, L, mode, npar, A.nchunks, A.p_total, A.p_ptot, A.p_key, A.p_bin,
                                 hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, st);

@@ -26,16 +26,93 @@
 
 namespace hd {
 
+// Slot order: slot t is position t of the level schedule (vertex lidx[t]).
+// slot_eoff: exclusive prefix of the slots' degrees (edges of slot t are
+// [slot_eoff[t], slot_eoff[t+1]) in slot-ordered CSR).  One workgroup.
+__global__ void __launch_bounds__(1024) k_ct_slots(int N, const int32_t* __restrict__ nptr,
+                                                   const int32_t* __restrict__ lidx, int32_t* __restrict__ slot_eoff) {
+    __shared__ int32_t s_part[1024];
+    const int tid = threadIdx.x;
+    int32_t carry = 0;
+    for (int base = 0; base < N; base += 1024) {
+        const int t = base + tid;
+        const int32_t d = (t < N) ? nptr[lidx[t] + 1] - nptr[lidx[t]] : 0;
+        s_part[tid] = d;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int32_t v = (tid >= off) ? s_part[tid - off] : 0;
+            __syncthreads();
+            s_part[tid] += v;
+            __syncthreads();
+        }
+        if (t < N) slot_eoff[t] = carry + s_part[tid] - d;
+        carry += s_part[1023];
+        __syncthreads();
+    }
+    if (tid == 0) slot_eoff[N] = carry;
+}
+
+// Geometry of every slot: the 2x2 system (Q0, Q1, Q3, det) and the edges
+// (neighbour, ex, ey, L^3) in slot-ordered CSR -- the same expressions in the
+// same order as the sweep used to evaluate them, so the sweep's arithmetic is
+// unchanged.  Edge storage holds ecap entries (6 N bounds a planar graph's
+// directed edges); an overflow is reported through *bad.
+__global__ void __launch_bounds__(256) k_ct_geom(int N, const double* __restrict__ pts, const int32_t* __restrict__ nptr,
+                                                 const int32_t* __restrict__ nidx, const int32_t* __restrict__ lidx,
+                                                 const int32_t* __restrict__ slot_eoff, int64_t ecap,
+                                                 double* __restrict__ slot_q, int32_t* __restrict__ e_q,
+                                                 double* __restrict__ e_geo, int32_t* __restrict__ bad) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int p = lidx[t];
+    const double px = pts[2 * p], py = pts[2 * p + 1];
+    double Q0 = 0, Q1 = 0, Q3 = 0;
+    int64_t e = slot_eoff[t];
+    if ((int64_t)slot_eoff[N] > ecap) {
+        if (t == 0) atomicAdd(bad, 1 << 20);
+        return;
+    }
+    for (int jj = nptr[p]; jj < nptr[p + 1]; jj++, e++) {
+        const int q = nidx[jj];
+        const double ex = pts[2 * q] - px;
+        const double ey = pts[2 * q + 1] - py;
+        const double L = sqrt(ex * ex + ey * ey);
+        const double L3 = L * L * L;
+        Q0 += 4 * ex * ex / L3;
+        Q1 += 4 * ex * ey / L3;
+        Q3 += 4 * ey * ey / L3;
+        e_q[e] = q;
+        e_geo[3 * e] = ex;
+        e_geo[3 * e + 1] = ey;
+        e_geo[3 * e + 2] = L3;
+    }
+    const double Q2 = Q1;
+    slot_q[4 * t] = Q0;
+    slot_q[4 * t + 1] = Q1;
+    slot_q[4 * t + 2] = Q3;
+    slot_q[4 * t + 3] = Q0 * Q3 - Q1 * Q2;
+}
+
+// dens (N, Z) -> densT (Z, N): each slice contiguous
+__global__ void __launch_bounds__(256) k_transpose_nz(const double* __restrict__ a, int N, int Z, double* __restrict__ b) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * Z) return;
+    const int64_t p = t / Z, z = t % Z;
+    b[z * N + p] = a[t];
+}
+
 template <bool LDS_Y>
-__global__ void __launch_bounds__(64) k_ct_gradients(const double* __restrict__ dens, int N, int Z,
-                                                     const double* __restrict__ pts, const int32_t* __restrict__ nptr,
-                                                     const int32_t* __restrict__ nidx, const int32_t* __restrict__ lptr,
-                                                     const int32_t* __restrict__ lidx, int n_levels, double tol,
-                                                     int maxiter, double* __restrict__ grad_g,
+__global__ void __launch_bounds__(64) k_ct_gradients(const double* __restrict__ densT, int N, int Z,
+                                                     const int32_t* __restrict__ lidx,
+                                                     const int32_t* __restrict__ slot_eoff,
+                                                     const double* __restrict__ slot_q, const int32_t* __restrict__ e_q,
+                                                     const double* __restrict__ e_geo, const int32_t* __restrict__ lptr,
+                                                     int n_levels, double tol, int maxiter, double* __restrict__ grad_g,
                                                      int32_t* __restrict__ iters_out) {
     extern __shared__ double s_y[];
     const int z = blockIdx.x;
     const int lane = threadIdx.x;
+    const double* frow = densT + (int64_t)z * N;
     double* y = LDS_Y ? s_y : (grad_g + (int64_t)z * N * 2);
     for (int i = lane; i < 2 * N; i += 64) y[i] = 0.0;
     __syncthreads();
@@ -46,25 +123,21 @@ __global__ void __launch_bounds__(64) k_ct_gradients(const double* __restrict__ 
             const int l0 = lptr[lev], l1 = lptr[lev + 1];
             for (int t = l0 + lane; t < l1; t += 64) {
                 const int p = lidx[t];
-                const double px = pts[2 * p], py = pts[2 * p + 1];
-                const double f1 = dens[(int64_t)p * Z + z];
-                double Q0 = 0, Q1 = 0, Q3 = 0, s0 = 0, s1 = 0;
-                for (int jj = nptr[p]; jj < nptr[p + 1]; jj++) {
-                    const int q = nidx[jj];
-                    const double ex = pts[2 * q] - px;
-                    const double ey = pts[2 * q + 1] - py;
-                    const double L = sqrt(ex * ex + ey * ey);
-                    const double L3 = L * L * L;
-                    const double f2 = dens[(int64_t)q * Z + z];
+                const int e0 = slot_eoff[t], e1 = slot_eoff[t + 1];
+                const double f1 = frow[p];
+                const double Q0 = slot_q[4 * t], Q1 = slot_q[4 * t + 1], Q3 = slot_q[4 * t + 2];
+                const double det = slot_q[4 * t + 3];
+                double s0 = 0, s1 = 0;
+#pragma unroll 4
+                for (int e = e0; e < e1; e++) {
+                    const int q = e_q[e];
+                    const double ex = e_geo[3 * e], ey = e_geo[3 * e + 1], L3 = e_geo[3 * e + 2];
+                    const double f2 = frow[q];
                     const double df2 = -ex * y[2 * q] - ey * y[2 * q + 1];
-                    Q0 += 4 * ex * ex / L3;
-                    Q1 += 4 * ex * ey / L3;
-                    Q3 += 4 * ey * ey / L3;
                     s0 += (6 * (f1 - f2) - 2 * df2) * ex / L3;
                     s1 += (6 * (f1 - f2) - 2 * df2) * ey / L3;
                 }
                 const double Q2 = Q1;
-                const double det = Q0 * Q3 - Q1 * Q2;
                 const double r0 = (Q3 * s0 - Q1 * s1) / det;
                 const double r1 = (-Q2 * s0 + Q0 * s1) / det;
                 double a0 = fabs(y[2 * p] + r0), a1 = fabs(y[2 * p + 1] + r1);
@@ -254,7 +327,9 @@ extern "C" int hd_gemm_f64(const double* A, const double* B, double* C, int m, i
 
 extern "C" size_t hd_scatter_workspace_size(int M, int Q, int Z, int G) {
     size_t N = (size_t)M * Q;
-    return align256(N * Z * 2 * 8) + 2 * align256((size_t)G * G * Z * 8) + 256 + 256;
+    const size_t ecap = 6 * N;
+    return align256(N * Z * 2 * 8) + 2 * align256((size_t)G * G * Z * 8) + align256(N * Z * 8) +   // grad, D, Y, densT
+           align256((N + 1) * 4) + align256(N * 4 * 8) + align256(ecap * 4) + align256(ecap * 3 * 8) + 256;
 }
 
 extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double* pts, const int32_t* nbr_ptr,
@@ -276,9 +351,21 @@ extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double
     double* grad = (double*)p;  p += align256((size_t)N * Z * 2 * 8);
     double* Dm = (double*)p;    p += align256((size_t)G * G * Z * 8);
     double* Y = (double*)p;     p += align256((size_t)G * G * Z * 8);
+    double* densT = (double*)p; p += align256((size_t)N * Z * 8);
+    const int64_t ecap = 6 * (int64_t)N;
+    int32_t* slot_eoff = (int32_t*)p;  p += align256((size_t)(N + 1) * 4);
+    double* slot_q = (double*)p;       p += align256((size_t)N * 4 * 8);
+    int32_t* e_q = (int32_t*)p;        p += align256((size_t)ecap * 4);
+    double* e_geo = (double*)p;
     int32_t* bad = iters_out + Z;   // iters_out has Z + 1 entries
 
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, bad, 1);
+    hipLaunchKernelGGL(k_ct_slots, dim3(1), dim3(1024), 0, st, N, nbr_ptr, levels_idx, slot_eoff);
+    hipLaunchKernelGGL(k_ct_geom, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, st, N, pts, nbr_ptr, nbr_idx, levels_idx,
+                       slot_eoff, ecap, slot_q, e_q, e_geo, bad);
+    hipLaunchKernelGGL(k_transpose_nz, dim3((unsigned)cdiv((int64_t)N * Z, 256)), dim3(256), 0, st, dens, N, Z, densT);
+    int rc0 = check_launch("k_ct_geom");
+    if (rc0) return rc0;
     // solution vector in LDS when it fits
     int dev = 0;
     hipGetDevice(&dev);
@@ -289,11 +376,11 @@ extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double
         if (y_bytes > 65536)
             hipFuncSetAttribute((const void*)k_ct_gradients<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)y_bytes);
-        hipLaunchKernelGGL(k_ct_gradients<true>, dim3(Z), dim3(64), y_bytes, st, dens, N, Z, pts, nbr_ptr, nbr_idx,
-                           levels_ptr, levels_idx, n_levels, tol, maxiter, grad, iters_out);
+        hipLaunchKernelGGL(k_ct_gradients<true>, dim3(Z), dim3(64), y_bytes, st, densT, N, Z, levels_idx, slot_eoff,
+                           slot_q, e_q, e_geo, levels_ptr, n_levels, tol, maxiter, grad, iters_out);
     } else {
-        hipLaunchKernelGGL(k_ct_gradients<false>, dim3(Z), dim3(64), 0, st, dens, N, Z, pts, nbr_ptr, nbr_idx,
-                           levels_ptr, levels_idx, n_levels, tol, maxiter, grad, iters_out);
+        hipLaunchKernelGGL(k_ct_gradients<false>, dim3(Z), dim3(64), 0, st, densT, N, Z, levels_idx, slot_eoff, slot_q,
+                           e_q, e_geo, levels_ptr, n_levels, tol, maxiter, grad, iters_out);
     }
     int rc = check_launch("k_ct_gradients");
     if (rc) return rc;

@@ -1,0 +1,357 @@
+"""Library generation over a parameter space, on the device path.
+
+Mirrors holodeck.librarian: the parameter distributions and the Latin-hypercube
+parameter space (librarian/lib_tools.py:34-465, 467-600), the classic
+phenomenological space (librarian/param_spaces_classic.py:9-108), ``run_model``
+(lib_tools.py:714-842) and the combined library file (librarian/combine.py:
+86-270; written as ``.npz`` with the same dataset and attribute names, since
+h5py is not a dependency here).
+
+Each sample is one Semi_Analytic_Model + Fixed_Time_2PL_SAM run on the GPU:
+density (with M-Mbulge scatter) -> hardening norms -> evolution -> number ->
+strain -> loud-source split (``hc_ss``, ``hc_bg``) and, as the reference's
+``gwb_flag``, an independent Poisson GWB (``gwb``).  Samples may have different
+grid shapes; they run back to back on one stream, every array stays in HBM
+until the sample's outputs are copied out.
+"""
+import abc
+import time
+
+import numpy as np
+import scipy.stats
+
+from .constants import GYR, PC, YR
+from . import host_relations, hardening, utils
+from . import sams
+from .sams.sam import Semi_Analytic_Model
+
+DEF_PTA_DUR = 16.03        # [yr]  (librarian/__init__.py)
+DEF_NUM_FBINS = 40
+DEF_NUM_REALS = 100
+DEF_NUM_LOUDEST = 5
+
+
+# ---------------------------------------------------------------------------
+# parameter distributions (lib_tools.py:467-600)
+# ---------------------------------------------------------------------------
+class _Param_Dist(abc.ABC):
+    """Maps uniform [0, 1] values to parameter values (usually an inverse CDF)."""
+
+    def __init__(self, name, default=None, clip=None):
+        if clip is not None:
+            assert len(clip) == 2
+        self._clip = clip
+        self._name = name
+        self._default = default
+
+    def __call__(self, xx):
+        rv = self._dist_func(xx)
+        if self._clip is not None:
+            rv = np.clip(rv, *self._clip)
+        return rv
+
+    @abc.abstractmethod
+    def _dist_func(self, xx):
+        pass
+
+    @property
+    def extrema(self):
+        return self(np.asarray([0.0, 1.0]))
+
+    @property
+    def name(self):
+        return self._name
+
+    @property
+    def default(self):
+        """The fixed default if one was given, else the value at uniform 0.5."""
+        if self._default is not None:
+            return self._default
+        return self(0.5)
+
+
+class PD_Uniform(_Param_Dist):
+    def __init__(self, name, lo, hi, **kwargs):
+        super().__init__(name, **kwargs)
+        self._lo = lo
+        self._hi = hi
+
+    def _dist_func(self, xx):
+        return self._lo + (self._hi - self._lo) * np.asarray(xx)
+
+
+class PD_Uniform_Log(_Param_Dist):
+    def __init__(self, name, lo, hi, **kwargs):
+        super().__init__(name, **kwargs)
+        assert lo > 0.0 and hi > 0.0
+        self._lo_log10 = np.log10(lo)
+        self._hi_log10 = np.log10(hi)
+
+    def _dist_func(self, xx):
+        return np.power(10.0, self._lo_log10 + (self._hi_log10 - self._lo_log10) * np.asarray(xx))
+
+
+class PD_Normal(_Param_Dist):
+    """Normal distribution with the given mean and standard deviation (use ``clip`` against extremes)."""
+
+    def __init__(self, name, mean, stdev, clip=None, **kwargs):
+        assert stdev > 0.0
+        super().__init__(name, clip=clip, **kwargs)
+        self._mean = mean
+        self._stdev = stdev
+        self._frozen = scipy.stats.norm(loc=mean, scale=stdev)
+
+    def _dist_func(self, xx):
+        return self._frozen.ppf(xx)
+
+
+# ---------------------------------------------------------------------------
+# parameter spaces (lib_tools.py:34-465, param_spaces_classic.py)
+# ---------------------------------------------------------------------------
+class _Param_Space(abc.ABC):
+    """A parameter space: distributions per varied parameter, Latin-hypercube samples, and the
+    SAM/hardening builders for a parameter dictionary (unvaried parameters from ``DEFAULTS``)."""
+
+    DEFAULTS = {}
+
+    def __init__(self, parameters, nsamples=None, sam_shape=None, seed=None):
+        parameters = list(parameters)
+        if len(parameters) == 0 or not all(isinstance(p, _Param_Dist) for p in parameters):
+            raise ValueError("`parameters` must be a non-empty list of `_Param_Dist` instances!")
+        self._parameters = parameters
+        self.param_names = [p.name for p in parameters]
+        self._nparameters = len(parameters)
+        self._nsamples = nsamples
+        self._seed = seed
+        self.sam_shape = sam_shape
+        if nsamples is None:
+            self._uniform_samples = None
+            self.param_samples = None
+        else:
+            lhc = scipy.stats.qmc.LatinHypercube(d=self._nparameters, strength=1, seed=seed)
+            self._uniform_samples = lhc.random(n=nsamples)                        # (S, D)
+            self.param_samples = np.zeros_like(self._uniform_samples)
+            for ii, param in enumerate(parameters):
+                self.param_samples[:, ii] = param(self._uniform_samples[:, ii])
+
+    @property
+    def name(self):
+        return self.__class__.__name__
+
+    @property
+    def nsamples(self):
+        return self._nsamples
+
+    @property
+    def nparameters(self):
+        return self._nparameters
+
+    @property
+    def extrema(self):
+        return np.array([p.extrema for p in self._parameters])
+
+    def param_dict(self, samp_num):
+        return {name: float(v) for name, v in zip(self.param_names, self.param_samples[samp_num])}
+
+    def normalized_params(self, vals):
+        """Uniform [0, 1] values (None: the parameter's default) -> parameter dictionary."""
+        vals = np.atleast_1d(vals)
+        if len(vals) != self._nparameters:
+            raise ValueError(f"{len(vals)=} != {self._nparameters=}")
+        return {p.name: (p.default if v is None else float(p(v))) for p, v in zip(self._parameters, vals)}
+
+    def model_for_params(self, params, sam_shape=None, cosmo=None):
+        if sam_shape is None:
+            sam_shape = self.sam_shape
+        settings = dict(self.DEFAULTS)
+        for name, value in params.items():
+            if name not in settings:
+                raise ValueError(f"`{name}` is not a parameter of {self.name}!")
+            settings[name] = value
+        sam = self._init_sam(sam_shape, settings, cosmo=cosmo)
+        hard = self._init_hard(sam, settings)
+        return sam, hard
+
+    def model_for_sample_number(self, samp_num, sam_shape=None, cosmo=None):
+        return self.model_for_params(self.param_dict(samp_num), sam_shape, cosmo=cosmo)
+
+    @classmethod
+    @abc.abstractmethod
+    def _init_sam(cls, sam_shape, params, cosmo=None):
+        pass
+
+    @classmethod
+    @abc.abstractmethod
+    def _init_hard(cls, sam, params):
+        pass
+
+
+class _PS_Classic_Phenom(_Param_Space):
+    """Classic phenomenological space of the 15-yr analysis (param_spaces_classic.py:9-98)."""
+
+    DEFAULTS = dict(
+        hard_time=3.0,          # [Gyr]
+        hard_sepa_init=1e4,     # [pc]
+        hard_rchar=100.0,       # [pc]
+        hard_gamma_inner=-1.0,
+        hard_gamma_outer=+2.5,
+        gsmf_phi0_log10=-2.77,
+        gsmf_phiz=-0.6,
+        gsmf_mchar0_log10=11.24,
+        gsmf_mcharz=0.11,
+        gsmf_alpha0=-1.21,
+        gsmf_alphaz=-0.03,
+        gpf_frac_norm_allq=0.025,
+        gpf_malpha=0.0,
+        gpf_qgamma=0.0,
+        gpf_zbeta=1.0,
+        gpf_max_frac=1.0,
+        gmt_norm=0.5,           # [Gyr]
+        gmt_malpha=0.0,
+        gmt_qgamma=-1.0,
+        gmt_zbeta=-0.5,
+        mmb_mamp_log10=8.69,
+        mmb_plaw=1.10,
+        mmb_scatter_dex=0.3,
+    )
+
+    @classmethod
+    def _init_sam(cls, sam_shape, params, cosmo=None):
+        gsmf = sams.GSMF_Schechter(phi0=params['gsmf_phi0_log10'], phiz=params['gsmf_phiz'],
+                                   mchar0_log10=params['gsmf_mchar0_log10'], mcharz=params['gsmf_mcharz'],
+                                   alpha0=params['gsmf_alpha0'], alphaz=params['gsmf_alphaz'])
+        gpf = sams.GPF_Power_Law(frac_norm_allq=params['gpf_frac_norm_allq'], malpha=params['gpf_malpha'],
+                                 qgamma=params['gpf_qgamma'], zbeta=params['gpf_zbeta'],
+                                 max_frac=params['gpf_max_frac'])
+        gmt = sams.GMT_Power_Law(time_norm=params['gmt_norm'] * GYR, malpha=params['gmt_malpha'],
+                                 qgamma=params['gmt_qgamma'], zbeta=params['gmt_zbeta'], cosmo=cosmo)
+        mmbulge = host_relations.MMBulge_KH2013(mamp_log10=params['mmb_mamp_log10'], mplaw=params['mmb_plaw'],
+                                                scatter_dex=params['mmb_scatter_dex'])
+        return Semi_Analytic_Model(gsmf=gsmf, gpf=gpf, gmt=gmt, mmbulge=mmbulge, shape=sam_shape, cosmo=cosmo)
+
+    @classmethod
+    def _init_hard(cls, sam, params):
+        return hardening.Fixed_Time_2PL_SAM(sam, params['hard_time'] * GYR, sepa_init=params['hard_sepa_init'] * PC,
+                                            rchar=params['hard_rchar'] * PC,
+                                            gamma_inner=params['hard_gamma_inner'],
+                                            gamma_outer=params['hard_gamma_outer'])
+
+
+class PS_Classic_Phenom_Uniform(_PS_Classic_Phenom):
+    """Classic uniform phenomenological space (param_spaces_classic.py:100-117)."""
+
+    def __init__(self, nsamples=None, sam_shape=None, seed=None):
+        parameters = [
+            PD_Uniform("gsmf_phi0_log10", -3.5, -1.5),
+            PD_Uniform("gsmf_mchar0_log10", 10.5, 12.5),   # [log10(Msol)]
+            PD_Uniform("mmb_mamp_log10", +7.6, +9.0),      # [log10(Msol)]
+            PD_Uniform("mmb_scatter_dex", +0.0, +0.9),
+            PD_Uniform("hard_time", 0.1, 11.0),            # [Gyr]
+            PD_Uniform("hard_gamma_inner", -1.5, +0.0),
+        ]
+        super().__init__(parameters, nsamples=nsamples, sam_shape=sam_shape, seed=seed)
+
+
+# ---------------------------------------------------------------------------
+# one model (lib_tools.py:714-842)
+# ---------------------------------------------------------------------------
+def run_model(sam, hard, pta_dur=DEF_PTA_DUR, nfreqs=DEF_NUM_FBINS, nreals=DEF_NUM_REALS, nloudest=DEF_NUM_LOUDEST,
+              gwb_flag=True, singles_flag=True, details_flag=False, params_flag=False, *, seed=0):
+    """Population and GW signatures of one model; same dictionary keys as the reference.
+
+    ``pta_dur`` is in years.  ``seed`` keys the Philox streams: the loud/background split uses
+    ``seed``, the independent Poisson GWB (``gwb_flag``) uses ``seed + 1``.
+    """
+    from . import engine, _lib
+    from . import device as D
+    if not any([gwb_flag, details_flag, singles_flag, params_flag]):
+        raise RuntimeError(f"No flags set!  {gwb_flag=} {details_flag=} {singles_flag=} {params_flag=}")
+    if not isinstance(hard, (hardening.Fixed_Time_2PL_SAM, hardening.Hard_GW)):
+        raise RuntimeError(f"`Fixed_Time_2PL_SAM` or `Hard_GW` must be used here!  Not {hard}!")
+    data = {}
+    fobs_cents, fobs_edges = utils.pta_freqs(dur=pta_dur * YR, num=nfreqs)
+    data['fobs_cents'] = fobs_cents
+    data['fobs_edges'] = fobs_edges
+    number, h2fdf, redz_final = sam._number_h2fdf_device(fobs_edges, hard)
+    if details_flag:
+        data['static_binary_density'] = sam.static_binary_density
+        data['number'] = D.to_host(number).reshape(tuple(n - 1 for n in sam.shape) + (nfreqs,))
+        data['redz_final'] = D.to_host(redz_final)
+    if singles_flag or params_flag:
+        from . import single_sources
+        nl = nloudest if singles_flag else 1
+        edges = [sam.mtot, sam.mrat, sam.redz, np.asarray(fobs_edges) / 2.0]
+        out = single_sources._ss_gws_redz_device(edges, redz_final, number, h2fdf, realize=nreals, loudest=nl,
+                                                 params=params_flag, seed=seed, counts=None, loud_mode='sorted',
+                                                 cosmo=sam._cosmo)
+        sam._check_scatter()
+        if params_flag:
+            hc_ss, hc_bg, sspar, bgpar = out
+            data['sspar'] = sspar
+            data['bgpar'] = bgpar
+        else:
+            hc_ss, hc_bg = out
+        if singles_flag:
+            data['hc_ss'] = hc_ss
+            data['hc_bg'] = hc_bg
+    if gwb_flag:
+        _, _, hc2, _ = engine.realize(number, h2fdf, int(nreals), 0, _lib.HD_SEL_NONE, seed=seed + 1)
+        data['gwb'] = np.sqrt(D.to_host(hc2))
+    return data
+
+
+# ---------------------------------------------------------------------------
+# a library (gen_lib.py:50-337 sample loop + combine.py:86-270 file layout)
+# ---------------------------------------------------------------------------
+def sample_shapes(nsamples, seed, mtot=(41, 91), mrat=(41, 81), redz=(41, 101)):
+    """Per-sample grid shapes (edge counts), uniform on the given inclusive ranges from one seed."""
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.integers(lo, hi + 1, size=nsamples) for lo, hi in (mtot, mrat, redz)], axis=1)
+
+
+def run_library(pspace, shapes=None, pta_dur=DEF_PTA_DUR, nfreqs=DEF_NUM_FBINS, nreals=DEF_NUM_REALS,
+                nloudest=DEF_NUM_LOUDEST, gwb_flag=True, singles_flag=True, params_flag=False, seed=0,
+                path=None, samples=None, cosmo=None, log_every=0):
+    """Run every sample of `pspace` (or the listed sample numbers) and collect the library arrays.
+
+    ``shapes``: (S, 3) grid edge counts per sample, or None for ``pspace.sam_shape``.
+    Returns the dictionary of the combined library file and writes it to ``path`` (``.npz``) if
+    given.  Sample ``i`` uses the Philox seed ``seed + 2 i``.
+    """
+    samples = np.arange(pspace.nsamples) if samples is None else np.asarray(samples)
+    fobs_cents, fobs_edges = utils.pta_freqs(dur=pta_dur * YR, num=nfreqs)
+    S = len(samples)
+    lib = {'fobs_cents': fobs_cents, 'fobs_edges': fobs_edges,
+           'sample_params': pspace.param_samples[samples].copy()}
+    if gwb_flag:
+        lib['gwb'] = np.zeros((S, nfreqs, nreals))
+    if singles_flag:
+        lib['hc_ss'] = np.zeros((S, nfreqs, nreals, nloudest))
+        lib['hc_bg'] = np.zeros((S, nfreqs, nreals))
+    if params_flag:
+        lib['sspar'] = np.zeros((S, 4, nfreqs, nreals, nloudest))
+        lib['bgpar'] = np.zeros((S, 7, nfreqs, nreals))
+    t0 = time.perf_counter()
+    for ii, snum in enumerate(samples):
+        shape = None if shapes is None else tuple(int(v) for v in shapes[ii])
+        sam, hard = pspace.model_for_sample_number(int(snum), sam_shape=shape, cosmo=cosmo)
+        data = run_model(sam, hard, pta_dur=pta_dur, nfreqs=nfreqs, nreals=nreals, nloudest=nloudest,
+                         gwb_flag=gwb_flag, singles_flag=singles_flag, params_flag=params_flag,
+                         seed=seed + 2 * int(snum))
+        for key in ('gwb', 'hc_ss', 'hc_bg', 'sspar', 'bgpar'):
+            if key in lib:
+                lib[key][ii] = data[key]
+        if log_every and (ii + 1) % log_every == 0:
+            dt = time.perf_counter() - t0
+            print(f"library: {ii + 1}/{S} samples, {dt:.1f} s ({(ii + 1) / dt:.2f} samples/s)", flush=True)
+    lib['param_names'] = np.array(pspace.param_names).astype('S')
+    lib['parameter_space_class_name'] = np.array(pspace.name)
+    if path is not None:
+        np.savez(path, **lib)
+    return lib
+
+
+def load_library(path):
+    """Read a library written by :func:`run_library` (no pickled objects)."""
+    with np.load(path, allow_pickle=False) as fh:
+        return {k: fh[k] for k in fh.files}

@@ -228,6 +228,8 @@ class Semi_Analytic_Model:
         it = self._scatter_iters
         if it is not None:
             bad = int(it[-1].item())
+            if bad >= (1 << 20):
+                raise ValueError("M-Mbulge scatter: the triangulation has more than 6 N directed edges")
             if bad:
                 raise ValueError(f"After 0th order interpolation, {bad} bad values remain!")
 

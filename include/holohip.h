@@ -31,6 +31,15 @@ const char *hd_last_error(void);
 /* Library ABI version. */
 int hd_version(void);
 
+/* Launch timing (no counterpart in the reference; for benchmarks).  While
+ * enabled, selected launch sequences record a HIP event pair on their stream:
+ * "hd_realize" (the whole hd_realize sequence) and "k_gz_draw" (its draw
+ * kernel).  hd_timing_enable clears earlier records; hd_timing_read waits for
+ * the recorded events and returns the summed milliseconds and the launch
+ * count for one name. */
+int hd_timing_enable(int on);
+int hd_timing_read(const char *name, double *total_ms, int *launches);
+
 /* ------------------------------------------------------------------------
  * Cosmology (flat LambdaCDM, no radiation).  Replaces the cosmopy/astropy
  * `holo.cosmo` instance (holodeck/__init__.py:81-86): the interpolation grids

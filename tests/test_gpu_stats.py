@@ -52,6 +52,49 @@ def test_poisson_sampler(lam, gpu):
         assert np.sum(x > 0) <= 5
 
 
+def test_per_bin_laws_inside_full_tiles(gpu):
+    """Every frequency holds 3000 bins with lambda log-uniform over 1e-4..3e5 (sparse bins, table
+    windows of every size, batches split by the LDS table, bins too wide for it); the strain is 1 on
+    one chosen bin and 0 elsewhere, so hc2bg[f] is that bin's own count.  Chi-square per frequency."""
+    print("[per-bin laws in full tiles]", flush=True)
+    from holodeck_amd import engine, _lib
+    from holodeck_amd import device as D
+    rng = np.random.default_rng(11)
+    B, F, R = 3000, 48, 4000
+    lam = 10.0 ** rng.uniform(-4, np.log10(3e5), size=(B, F))
+    sel = rng.integers(0, B, size=F)
+    # pin a spread of regimes on the selected bins
+    pinned = [3e-4, 0.05, 0.6, 0.999, 1.0, 2.5, 9.9, 10.0, 33.0, 120.0, 900.0, 4.2e3, 2.0e4, 9.0e4, 2.9e5]
+    for f, v in enumerate(pinned):
+        lam[sel[f], f] = v
+    h = np.zeros((B, F))
+    h[sel, np.arange(F)] = 1.0
+    _, _, bg, _ = engine.realize(D.to_dev(lam), D.to_dev(h), R, 0, _lib.HD_SEL_NONE, seed=77)
+    x = D.to_host(bg)
+    pvals = []
+    for f in range(F):
+        l = lam[sel[f], f]
+        xf = x[f]
+        assert np.all(xf == np.floor(xf)) and np.all(xf >= 0), f
+        assert abs(xf.mean() - l) < 6 * np.sqrt(l / R) + 1e-12, (f, l, xf.mean())
+        kmin, kmax = int(stats.poisson.ppf(1e-4, l)), int(stats.poisson.ppf(1 - 1e-4, l))
+        ks = np.arange(kmin, kmax + 1)
+        e = stats.poisson.pmf(ks, l) * R
+        ok = e > 20
+        if ok.sum() >= 2:
+            o = np.array([np.sum(xf == k) for k in ks], dtype=float)[ok]
+            chi2 = np.sum((o - e[ok]) ** 2 / e[ok])
+            pvals.append(stats.chi2.sf(chi2, ok.sum() - 1))
+    assert min(pvals) > 1e-3 / len(pvals), sorted(pvals)[:5]
+    # the background of all bins at once is Poisson(sum lambda) when h = 1 everywhere
+    _, _, bg1, _ = engine.realize(D.to_dev(lam[:, :4]), D.to_dev(np.ones((B, 4))), R, 0, _lib.HD_SEL_NONE, seed=78)
+    tot = D.to_host(bg1)
+    for f in range(4):
+        S = lam[:, f].sum()
+        assert abs(tot[f].mean() - S) < 6 * np.sqrt(S / R), (f, S, tot[f].mean())
+        assert abs(tot[f].var() / S - 1) < 6 * np.sqrt(2.0 / R)
+
+
 def test_normal_branch_above_threshold(gpu):
     lam = 3.0e10
     x = _draws(lam, 100000, seed=5)

@@ -50,6 +50,17 @@ def test_errors_are_reported_without_gpu():
         _lib.call("hd_realize_counts", 1, 1, None, 10, 2, 3, 99, 0, 0, None, 0, 1, 1, 1, None, None, 0, None)
 
 
+def test_launch_timing_switch_without_gpu():
+    """Timing records nothing until a timed launch runs; enabling clears, reading an unknown name gives zero."""
+    from holodeck_amd import _lib
+    _lib.timing_enable(True)
+    assert _lib.timing_read("hd_realize") == (0.0, 0)
+    assert _lib.timing_read("no_such_kernel") == (0.0, 0)
+    _lib.timing_enable(False)
+    with pytest.raises(_lib.HoloHipError, match="null argument"):
+        _lib.call("hd_timing_read", b"hd_realize", None, None)
+
+
 def test_pta_freqs_and_midpoints():
     from holodeck_amd import utils
     from holodeck_amd.constants import YR
