@@ -51,11 +51,20 @@ SIGNATURES = {
     "hd_realize": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _D, C.c_uint64, _P, _I32, _P, _P, _P,
                           _P, _P, _SZ, _P]),
     "hd_philox4x32_10": (None, [_P, _P, _P]),
+    "hd_mlp_gemm_bf16": (_I32, [_P, _I64, _I64, _P, _I64, _I64, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _I64, _P]),
+    "hd_mlp_colsum_bf16": (_I32, [_P, _I32, _I32, _P, _P]),
+    "hd_mlp_mse": (_I32, [_P, _P, _I64, _P, _P, _P]),
+    "hd_mlp_adam": (_I32, [_I64, _P, _P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float, _I32, _P, _P]),
+    "hd_f32_to_bf16": (_I32, [_P, _I64, _P, _P]),
 }
 
 HD_SEL_SORTED = 0
 HD_SEL_ARGMAX = 1
 HD_SEL_NONE = 2
+HD_MLP_EPI_STORE = 0
+HD_MLP_EPI_BIAS_SILU = 1
+HD_MLP_EPI_BIAS_BF16 = 2
+HD_MLP_EPI_SILU_BWD = 3
 HD_FLAG_SKIP_ZERO_H = 1
 HD_DENS_NPAR = 48
 

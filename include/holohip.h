@@ -197,6 +197,31 @@ int hd_realize(const double *number, const double *h2fdf, const int64_t *order, 
  * generator the realization kernels use. */
 void hd_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 
+/* ------------------------------------------------------------------------
+ * Learned surrogate (MLP: parameters -> log10 h_c percentiles).  No
+ * counterpart in the reference, whose emulator is the Gaussian process of
+ * holodeck/gps/gp_utils.py; the surrogate sits beside it.  BF16 operands
+ * (passed as void*: 2-byte bfloat16), FP32 accumulation and master weights.
+ * ---------------------------------------------------------------------- */
+#define HD_MLP_EPI_STORE 0     /* out_f32 = acc (+ bias)                          */
+#define HD_MLP_EPI_BIAS_SILU 1 /* z = acc + bias -> out_f32 (opt.); silu(z) -> out_bf16 */
+#define HD_MLP_EPI_BIAS_BF16 2 /* acc (+ bias) -> out_bf16 (and out_f32 if given)  */
+#define HD_MLP_EPI_SILU_BWD 3  /* acc * silu'(zin) -> out_bf16                     */
+/* C (M x N) = A (M x K) . B^T, B given as (N x K); element (r, k) of A at
+ * A[r*sam + k*sak], of B at B[r*sbn + k*sbk]; C row stride ldo. */
+int hd_mlp_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbn, int64_t sbk, int M, int N,
+                     int K, int epi, const float *bias, const float *zin, float *out_f32, void *out_bf16,
+                     int64_t ldo, void *stream);
+/* out[n] = sum_m a[m, n] (bias gradients) */
+int hd_mlp_colsum_bf16(const void *a, int M, int N, float *out, void *stream);
+/* dout = 2 (y - t) / n (bf16); *loss += mean squared error (FP32, accumulated) */
+int hd_mlp_mse(const float *y, const float *t, int64_t n, void *dout, float *loss, void *stream);
+/* Adam step `step` (>= 1) on FP32 master weights w with gradient g and moments
+ * m, v; writes the bf16 copy w_bf16 when not NULL */
+int hd_mlp_adam(int64_t n, float *w, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+                int step, void *w_bf16, void *stream);
+int hd_f32_to_bf16(const float *a, int64_t n, void *b, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
