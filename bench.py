@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nreals", type=int, default=500)
+    ap.add_argument("--streams", type=int, default=3,
+                    help="independent spectra in flight on this many HIP streams (consecutive steps alternate)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy a 1 GiB device buffer once (reference kernel for PMC byte "
                          "counters)")
@@ -116,16 +118,22 @@ def main():
         hard = configs.make_hard(spec, sam)
         return sam.gwb_device(fe, hard, R, L, seed=seed)
 
+    # warm-up on the default stream: builds the cached scatter plan and device tables that the
+    # streams then only read
     for w in range(args.warmup):
         out = step(1000 + w)
     torch.cuda.synchronize()
+    streams = [torch.cuda.Stream() for _ in range(max(1, args.streams))]   # default priority
     engine.EVENTS = {}
     _lib.timing_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    outs = []
     for s in range(args.steps):
-        out = step(s)
+        with torch.cuda.stream(streams[s % len(streams)]):
+            outs.append(step(s))
     torch.cuda.synchronize()
+    out = outs[-1]
     dt = time.perf_counter() - t0
     stages = engine.stage_times_ms()
     engine.EVENTS = None
@@ -161,7 +169,8 @@ def main():
                                 "40 frequencies i/T (T=16.03 yr), flat LCDM H0=67.7 Om=0.31, GSMF Schechter + "
                                 "GPF/GMT power laws, KH2013 M-Mbulge with 0.28 dex scatter, Fixed_Time_2PL_SAM "
                                 "(1 Gyr, -1, +2.5, 100 pc, 1e4 pc), 500 realizations, 10 loudest"),
-                   "grid": [M, Q, Z], "nfreq": F, "nreals": R, "nloudest": L},
+                   "grid": [M, Q, Z], "nfreq": F, "nreals": R, "nloudest": L,
+                   "streams": len(streams)},
         "cell_realizations_per_s": B * F * R * value,
         "stages_ms": stages,
         "roofline": {"bound": "hbm", "kernel": "hd_realize (k_gz_prep + k_gz_draw + k_gz_big + k_realize_merge)",

@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
         r.word = word;
         r.klo = klo;
         r.n = nwin;
-        r.off = on + xn - nwin;   // exclusive prefix of window lengths
+        r.off = on + xn - nwin;   // exclusive prefix of the window lengths
         r.pad = 0;
         A.drec[rreg + i] = r;
         A.rec_end[rreg + i] = r.off + nwin;
@@ -260,6 +260,10 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
 // first L slots is cached.  The sorted walk ranks instances by their position
 // in the order (32-bit key; the bin is recovered from the order at the end);
 // argmax ranks by (h descending, bin ascending).
+// LM = 1, 5, 10 are instantiated for exactly L = LM (the slot of the worst
+// kept instance is then a compile-time constant); LM = 4, 16 serve any L up to
+// LM with a run-time L.
+__host__ __device__ constexpr bool gz_exact(int LM) { return LM != 4 && LM != 16; }
 template <int MODE, int LM>
 struct LoudList;
 
@@ -274,6 +278,7 @@ struct LoudList<HD_SEL_SORTED, LM> {
     }
     __device__ __forceinline__ bool admits(uint64_t word, double) const { return (uint32_t)word < wk; }
     __device__ __forceinline__ void insert(uint64_t word, double, int L) {
+        if (gz_exact(LM)) L = LM;
         uint32_t key = (uint32_t)word;
 #pragma unroll
         for (int i = 0; i < LM; i++) {
@@ -290,6 +295,7 @@ struct LoudList<HD_SEL_SORTED, LM> {
         return k[j] == 0xffffffffu ? GZ_KEY_NONE : (uint64_t)k[j];
     }
     __device__ __forceinline__ void load(const uint64_t* pk, const int32_t*, int L, int64_t stride) {
+        if (gz_exact(LM)) L = LM;
 #pragma unroll
         for (int i = 0; i < LM; i++) {
             const uint64_t key = (i < L) ? pk[i * stride] : GZ_KEY_NONE;
@@ -323,6 +329,7 @@ struct LoudList {
         return (key < wk) || (key == wk && bin < wb);
     }
     __device__ __forceinline__ void insert(uint64_t word, double h, int L) {
+        if (gz_exact(LM)) L = LM;
         uint64_t key = gz_hkey(h);
         int32_t bin = (int32_t)(word >> 32);
 #pragma unroll
@@ -344,6 +351,7 @@ struct LoudList {
     }
     __device__ __forceinline__ uint64_t key_at(int j) const { return k[j]; }
     __device__ __forceinline__ void load(const uint64_t* pk, const int32_t* pb, int L, int64_t stride) {
+        if (gz_exact(LM)) L = LM;
 #pragma unroll
         for (int i = 0; i < LM; i++) {
             k[i] = (i < L) ? pk[i * stride] : GZ_KEY_NONE;
@@ -409,12 +417,36 @@ __device__ __noinline__ void gz_build(double* tab, double lam, int klo, int n, i
         if (lane >= off) x += y;
     }
     const double carry = x - local;
-    if (m > 0 && lane > 0)
-        for (int j = 0; j < m; j++) tab[j0 + j] += carry;
+    double last = -1.0;
+    if (m > 0) {
+        if (lane > 0)
+            for (int j = 0; j < m; j++) tab[j0 + j] += carry;
+        last = tab[j0 + m - 1];
+    }
+    // The carries are rounded independently of the segment sums, so two
+    // neighbouring segments can meet out of order by an ulp; raise each
+    // segment to the largest value before it, which makes the table exactly
+    // non-decreasing (so the inversion is a monotone map of the uniform).
+    double mx = last;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(mx, off, 64);
+        if (lane >= off) mx = fmax(mx, y);
+    }
+    const double before = __shfl_up(mx, 1, 64);
+    if (m > 0 && lane > 0 && tab[j0] < before)
+        for (int j = 0; j < m; j++) tab[j0 + j] = fmax(tab[j0 + j], before);
 }
 
-// Inversion: the smallest k in the window with CDF(k) > u; a uniform above the
-// table's last value (rounding of the CDF near 1) returns the window's end.
+// Uniform in [0, 1) from 52 random bits: a double in [1, 2) minus 1 (exact).
+__device__ __forceinline__ double gz_u52(uint32_t hi, uint32_t lo) {
+    const uint64_t bits = 0x3FF0000000000000ull | ((uint64_t)(hi & 0xFFFFFu) << 32) | lo;
+    return __longlong_as_double((long long)bits) - 1.0;
+}
+
+// Inversion: the smallest k in the window with CDF(k) > u (branch-free binary
+// search); a uniform above the table's last value (CDF rounding near 1)
+// returns the window's end.
 __device__ __forceinline__ double gz_table_draw(const double* tab, int klo, int n, double u) {
     int base = 0, len = n;
     while (len > 1) {   // invariant: the answer is in [base, base + len]
@@ -438,7 +470,7 @@ __device__ __forceinline__ double gz_fallback(uint32_t a, uint32_t f, uint32_t r
 
 template <int MODE, int LM, int NP>
 __device__ __forceinline__ void gz_store(const GapArgs& A, const Acc<MODE, LM, NP>& acc, int64_t pr) {
-    const int F = A.F, R = A.R, L = A.L;
+    const int F = A.F, R = A.R, L = gz_exact(LM) ? LM : A.L;
     A.p_total[pr] = acc.tot;
 #pragma unroll
     for (int k = 0; k < NP; k++)
@@ -504,13 +536,21 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                     for (int j = tid; j < ns; j += bs) scum[j] = A.s_cum[reg + j];
                 __syncthreads();
                 if (active) {
+                    // One Philox block per record pair (2i, 2i+1) of the tile: words (x, y) for the
+                    // even record, (z, w) for the odd one; counter (tile, f, r, tag 5 | pair).
+                    int blk_pair = -1;
+                    U4 blk = {0u, 0u, 0u, 0u};
                     for (int i = c; i < c1; i++) {
                         const DRec rec = A.drec[rreg + i];
                         if (i < nd) {
                             if (rec.n == 0) continue;   // no table: drawn by k_gz_big
-                            const uint32_t bin = (uint32_t)(rec.word >> 32);
-                            const U4 a = philox4x32_10(U4{bin, (uint32_t)f, (uint32_t)r, 1u << 16}, k0, k1);
-                            const double n = gz_table_draw(tab + (rec.off - base), rec.klo, rec.n, u53(a.x, a.y));
+                            if ((i >> 1) != blk_pair) {
+                                blk_pair = i >> 1;
+                                blk = philox4x32_10(U4{(uint32_t)tile, (uint32_t)f, (uint32_t)r,
+                                                       (5u << 16) | (uint32_t)blk_pair}, k0, k1);
+                            }
+                            const double u = (i & 1) ? gz_u52(blk.z, blk.w) : gz_u52(blk.x, blk.y);
+                            const double n = gz_table_draw(tab + (rec.off - base), rec.klo, rec.n, u);
                             acc.add(n, rec.h, rec.word, A.d_par + reg + i, pstride, A.npar, L);
                         } else {
                             // sparse bins: Poisson process on the tile's cumulative-lambda axis; the
@@ -518,7 +558,7 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                             // the positions from stream tag 2
                             const double Lam = scum[ns - 1];
                             const U4 a = philox4x32_10(U4{(uint32_t)tile, (uint32_t)f, (uint32_t)r, 4u << 16}, k0, k1);
-                            const int np = (int)gz_table_draw(tab + (rec.off - base), rec.klo, rec.n, u53(a.x, a.y));
+                            const int np = (int)gz_table_draw(tab + (rec.off - base), rec.klo, rec.n, gz_u52(a.x, a.y));
                             Stream st;
                             st.init((uint32_t)tile, (uint32_t)f, (uint32_t)r, 2u, A.seed);
                             for (int q = 0; q < np; q++) {
@@ -590,9 +630,10 @@ int rank_from_order_launch(const int64_t* order, int64_t B, int32_t* rank, hipSt
 static size_t gz_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int gz_block(int R) { return (int)min((int64_t)GZ_MAXBS, (int64_t)cdiv(R, 64) * 64); }
-// LDS table entries per workgroup: 16 per thread, at least 2048 (> any sparse
-// tile window), at most 7680 so that table + sparse axis stay within 64 KiB
-static int gz_cap(int bs) { return max(2048, min(7680, bs * 16)); }
+// LDS table capacity (doubles) of every draw workgroup: table + sparse axis =
+// 64 KiB.  Fixed, so that which bins get a table (and so every draw) does not
+// depend on the launch shape.
+static int gz_cap(int) { return 7680; }
 
 static size_t gz_ws(int64_t B, int F, int R, int L, int npar) {
     const int64_t nt = cdiv(B, GZ_TILE);
@@ -626,7 +667,7 @@ static int gz_launch(const GapArgs& A, dim3 grid, int bs, size_t lds, hipStream_
 template <int MODE, int LM>
 static int gz_launch_np(const GapArgs& A, int npar, dim3 grid, int bs, size_t lds, hipStream_t st) {
     if (npar == 0) return gz_launch<MODE, LM, 0>(A, grid, bs, lds, st);
-    if (npar <= 3) return gz_launch<MODE, LM, 3>(A, grid, bs, lds, st);
+    if (MODE == HD_SEL_ARGMAX && npar <= 3) return gz_launch<MODE, LM, 3>(A, grid, bs, lds, st);
     return gz_launch<MODE, LM, 7>(A, grid, bs, lds, st);
 }
 
@@ -708,11 +749,13 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     const dim3 grid(A.nchunks, F);
     const size_t lds = (size_t)(A.cap + GZ_TILE) * sizeof(double);
     if (mode == HD_SEL_SORTED) {
-        if (L <= 1) rc = gz_launch_np<HD_SEL_SORTED, 1>(A, npar, grid, bs, lds, st);
+        if (L == 1) rc = gz_launch_np<HD_SEL_SORTED, 1>(A, npar, grid, bs, lds, st);
+        else if (L == 5) rc = gz_launch_np<HD_SEL_SORTED, 5>(A, npar, grid, bs, lds, st);
+        else if (L == 10) rc = gz_launch_np<HD_SEL_SORTED, 10>(A, npar, grid, bs, lds, st);
         else if (L <= 4) rc = gz_launch_np<HD_SEL_SORTED, 4>(A, npar, grid, bs, lds, st);
         else rc = gz_launch_np<HD_SEL_SORTED, 16>(A, npar, grid, bs, lds, st);
     } else if (mode == HD_SEL_ARGMAX) {
-        if (L <= 1) rc = gz_launch_np<HD_SEL_ARGMAX, 1>(A, npar, grid, bs, lds, st);
+        if (L == 1) rc = gz_launch_np<HD_SEL_ARGMAX, 1>(A, npar, grid, bs, lds, st);
         else if (L <= 4) rc = gz_launch_np<HD_SEL_ARGMAX, 4>(A, npar, grid, bs, lds, st);
         else rc = gz_launch_np<HD_SEL_ARGMAX, 16>(A, npar, grid, bs, lds, st);
     } else {
