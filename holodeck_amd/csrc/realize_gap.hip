@@ -56,18 +56,30 @@ constexpr int GZ_TILE = 512;          // bins per tile (prep workgroup size)
 constexpr int GZ_REC = GZ_TILE + 1;   // record slots per tile (+1 sparse pseudo-record)
 constexpr int GZ_TILES_PER_CHUNK = 16;
 constexpr int GZ_MAXBS = 512;         // draw workgroup: one realization per thread
+constexpr int GZ_BATCH_REC = 256;     // at most this many records per LDS batch (their headers are staged)
 constexpr int GZ_XCDS = 8;            // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr uint64_t GZ_KEY_NONE = ~0ull;
 constexpr double GZ_TAILB = 38.816242111356935;   // 56 ln 2: each dropped tail < 2^-56
+#ifndef HD_GZ_TAU
+#define HD_GZ_TAU 1.0   // lambda below which a bin joins its tile's sparse Poisson process
+#endif
 
 // dense record (and the sparse pseudo-record at index nd)
 struct DRec {
     double lam, h;
     uint64_t word;    // bin << 32 | rank
+    double p0;        // Poisson pmf at klo
     int32_t klo;      // first count of the CDF window
     int32_t n;        // window length (0: no table, use the fallback sampler)
     int32_t off;      // offset of the window in the tile's concatenated tables
     int32_t pad;
+};
+
+// header of a record staged in LDS for the draw loop (off relative to the batch's table)
+struct BRec {
+    double h;
+    uint64_t word;
+    int32_t klo, n, off, pad;
 };
 
 struct GapArgs {
@@ -219,6 +231,7 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
         r.word = word;
         r.klo = klo;
         r.n = nwin;
+        r.p0 = (nwin > 0) ? gz_pmf(lam, log(lam), (double)klo) : 0.0;
         r.off = on + xn - nwin;   // exclusive prefix of the window lengths
         r.pad = 0;
         A.drec[rreg + i] = r;
@@ -240,6 +253,7 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
             r.h = 0.0;
             r.word = 0;
             gz_window(cum, A.cap, r.klo, r.n);
+            r.p0 = (r.n > 0) ? gz_pmf(cum, log(cum), (double)r.klo) : 0.0;
             r.off = ntot;
             r.pad = 0;
             A.drec[rreg + nd] = r;
@@ -391,23 +405,44 @@ struct Acc {
     }
 };
 
-// Build the CDF window of one record into tab[0, n) with one wave: each lane
-// takes a contiguous segment, starts from the exact pmf and recurses
-// p(k+1) = p(k) lam / (k+1); segment sums are joined by a wave scan.
-__device__ __noinline__ void gz_build(double* tab, double lam, int klo, int n, int lane) {
+// 1/x for x >= 1: hardware reciprocal refined by two Newton steps (about an
+// ulp, against ~10 instructions for the correctly rounded divide).
+__device__ __forceinline__ double gz_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Build the CDF window of one record into tab[0, n) with one wave.  Each lane
+// takes a contiguous segment; p(klo) = p0 comes from the prep kernel, and the
+// pmf along the window follows p(k+1) = p(k) lam / (k+1): the lanes' segment
+// products of these ratios are joined by a wave prefix product, then each lane
+// runs its segment's recursion and partial sums, which a wave prefix sum joins.
+// (Relative error of the tables about n ulp, ~1e-12 for the widest window.)
+__device__ __noinline__ void gz_build(double* tab, double lam, int klo, int n, double p0, int lane) {
     const int s = (n + 63) >> 6;
     const int j0 = lane * s;
     const int m = min(max(n - j0, 0), s);
-    const double loglam = log(lam);
+    const double kb = (double)(klo + j0);
+    double prod = 1.0;
+    for (int j = 0; j < m; j++) prod *= lam * gz_rcp(kb + (double)(j + 1));
+    double px = prod;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(px, off, 64);
+        if (lane >= off) px *= y;
+    }
+    double pre = __shfl_up(px, 1, 64);   // product over the lanes before this one
+    if (lane == 0) pre = 1.0;
     double local = 0.0;
     if (m > 0) {
-        double k = (double)(klo + j0);
-        double p = gz_pmf(lam, loglam, k);
+        double p = p0 * pre;                   // p(klo + j0)
         for (int j = 0; j < m; j++) {
             local += p;
             tab[j0 + j] = local;
-            k += 1.0;
-            p = p * lam / k;
+            p *= lam * gz_rcp(kb + (double)(j + 1));
         }
     }
     double x = local;
@@ -489,12 +524,14 @@ __device__ __forceinline__ void gz_store(const GapArgs& A, const Acc<MODE, LM, N
 template <int MODE, int LM, int NP>
 __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
     extern __shared__ double gz_lds[];
+    __shared__ int s_next;             // next record of the batch to build
     double* tab = gz_lds;              // A.cap doubles
     double* scum = gz_lds + A.cap;     // GZ_TILE doubles
+    BRec* brec = (BRec*)(scum + GZ_TILE);   // GZ_BATCH_REC staged record headers
     const int f = blockIdx.y;
     const int chunk = blockIdx.x;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int bs = blockDim.x, nw = bs >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int bs = blockDim.x;
     const int F = A.F, R = A.R, L = A.L;
     const int t0 = chunk * GZ_TILES_PER_CHUNK;
     const int t1 = min(t0 + GZ_TILES_PER_CHUNK, A.ntiles);
@@ -527,10 +564,18 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                     if (adv < 64) break;
                 }
                 c1 = max(c1, c + 1);   // every window fits alone (n <= cap); keeps progress unconditional
+                c1 = min(c1, c + GZ_BATCH_REC);
+                if (tid == 0) s_next = c;
                 __syncthreads();   // the previous batch's draws are done with tab / scum
-                for (int i = c + wave; i < c1; i += nw) {
+                // waves take the batch's records one at a time (windows differ in length)
+                for (;;) {
+                    int i = 0;
+                    if (lane == 0) i = atomicAdd(&s_next, 1);
+                    i = __shfl(i, 0, 64);
+                    if (i >= c1) break;
                     const DRec rec = A.drec[rreg + i];
-                    if (rec.n > 0) gz_build(tab + (rec.off - base), rec.lam, rec.klo, rec.n, lane);
+                    if (lane == 0) brec[i - c] = BRec{rec.h, rec.word, rec.klo, rec.n, rec.off - base, 0};
+                    if (rec.n > 0) gz_build(tab + (rec.off - base), rec.lam, rec.klo, rec.n, rec.p0, lane);
                 }
                 if (c1 == nrec && ns > 0)
                     for (int j = tid; j < ns; j += bs) scum[j] = A.s_cum[reg + j];
@@ -541,7 +586,7 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                     int blk_pair = -1;
                     U4 blk = {0u, 0u, 0u, 0u};
                     for (int i = c; i < c1; i++) {
-                        const DRec rec = A.drec[rreg + i];
+                        const BRec rec = brec[i - c];
                         if (i < nd) {
                             if (rec.n == 0) continue;   // no table: drawn by k_gz_big
                             if ((i >> 1) != blk_pair) {
@@ -550,7 +595,7 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                                                        (5u << 16) | (uint32_t)blk_pair}, k0, k1);
                             }
                             const double u = (i & 1) ? gz_u52(blk.z, blk.w) : gz_u52(blk.x, blk.y);
-                            const double n = gz_table_draw(tab + (rec.off - base), rec.klo, rec.n, u);
+                            const double n = gz_table_draw(tab + rec.off, rec.klo, rec.n, u);
                             acc.add(n, rec.h, rec.word, A.d_par + reg + i, pstride, A.npar, L);
                         } else {
                             // sparse bins: Poisson process on the tile's cumulative-lambda axis; the
@@ -558,7 +603,7 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                             // the positions from stream tag 2
                             const double Lam = scum[ns - 1];
                             const U4 a = philox4x32_10(U4{(uint32_t)tile, (uint32_t)f, (uint32_t)r, 4u << 16}, k0, k1);
-                            const int np = (int)gz_table_draw(tab + (rec.off - base), rec.klo, rec.n, gz_u52(a.x, a.y));
+                            const int np = (int)gz_table_draw(tab + rec.off, rec.klo, rec.n, gz_u52(a.x, a.y));
                             Stream st;
                             st.init((uint32_t)tile, (uint32_t)f, (uint32_t)r, 2u, A.seed);
                             for (int q = 0; q < np; q++) {
@@ -630,10 +675,13 @@ int rank_from_order_launch(const int64_t* order, int64_t B, int32_t* rank, hipSt
 static size_t gz_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int gz_block(int R) { return (int)min((int64_t)GZ_MAXBS, (int64_t)cdiv(R, 64) * 64); }
-// LDS table capacity (doubles) of every draw workgroup: table + sparse axis =
-// 64 KiB.  Fixed, so that which bins get a table (and so every draw) does not
-// depend on the launch shape.
-static int gz_cap(int) { return 7680; }
+// LDS table capacity (doubles) of every draw workgroup: table + sparse axis +
+// staged record headers + the build counter stay within 64 KiB.  Fixed, so that which bins get
+// a table (and so every draw) does not depend on the launch shape.
+#ifndef HD_GZ_CAP
+#define HD_GZ_CAP 6392
+#endif
+static int gz_cap(int) { return HD_GZ_CAP; }
 
 static size_t gz_ws(int64_t B, int F, int R, int L, int npar) {
     const int64_t nt = cdiv(B, GZ_TILE);
@@ -715,7 +763,7 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     A.mode = mode;
     A.cap = gz_cap(bs);
     A.thresh = normal_threshold;
-    A.tau = fmin(1.0, normal_threshold);   // bins above the threshold are never sparse
+    A.tau = fmin(HD_GZ_TAU, normal_threshold);   // bins above the threshold are never sparse
     A.seed = seed;
     A.ntiles = (int)cdiv(B, GZ_TILE);
     A.nchunks = (int)cdiv(A.ntiles, GZ_TILES_PER_CHUNK);
@@ -747,7 +795,7 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     int rc = check_launch("k_gz_prep");
     if (rc) return rc;
     const dim3 grid(A.nchunks, F);
-    const size_t lds = (size_t)(A.cap + GZ_TILE) * sizeof(double);
+    const size_t lds = (size_t)(A.cap + GZ_TILE) * sizeof(double) + (size_t)GZ_BATCH_REC * sizeof(BRec);
     if (mode == HD_SEL_SORTED) {
         if (L == 1) rc = gz_launch_np<HD_SEL_SORTED, 1>(A, npar, grid, bs, lds, st);
         else if (L == 5) rc = gz_launch_np<HD_SEL_SORTED, 5>(A, npar, grid, bs, lds, st);
