@@ -11,8 +11,10 @@ Each sample is one Semi_Analytic_Model + Fixed_Time_2PL_SAM run on the GPU:
 density (with M-Mbulge scatter) -> hardening norms -> evolution -> number ->
 strain -> loud-source split (``hc_ss``, ``hc_bg``) and, as the reference's
 ``gwb_flag``, an independent Poisson GWB (``gwb``).  Samples may have different
-grid shapes; they run back to back on one stream, every array stays in HBM
-until the sample's outputs are copied out.
+grid shapes.  ``run_library`` runs them back to back with every output of
+``run_model``; ``gwb_library`` keeps only the GWB and overlaps samples on
+several HIP streams, with the scatter geometry of new grids built ahead on host
+threads.
 """
 import abc
 import time
@@ -349,6 +351,129 @@ def run_library(pspace, shapes=None, pta_dur=DEF_PTA_DUR, nfreqs=DEF_NUM_FBINS, 
     if path is not None:
         np.savez(path, **lib)
     return lib
+
+
+def gwb_library(pspace, shapes=None, pta_dur=DEF_PTA_DUR, nfreqs=DEF_NUM_FBINS, nreals=DEF_NUM_REALS, seed=0,
+                samples=None, cosmo=None, streams=3, chunk=256, reduce=None, reduce_key='targets', plan_workers=8,
+                lookahead=16, log_every=0):
+    """The independent Poisson GWB of every sample (``run_model``'s ``gwb_flag`` output alone),
+    with samples in flight on several HIP streams (:func:`gwb_for_params`).
+
+    Sample ``i`` uses the Philox seed ``seed + 2 i + 1``, so ``gwb`` equals :func:`run_library`'s
+    for the same ``seed``.  ``reduce``: None keeps ``gwb`` (S, F, R); else a function of one block's
+    (n, F, R) strain whose outputs are concatenated under ``reduce_key`` (e.g. percentile targets)
+    and ``gwb`` is not kept.  Returns the library dictionary plus ``samples_per_s`` (whole run,
+    wall clock).
+    """
+    samples = np.arange(pspace.nsamples) if samples is None else np.asarray(samples)
+    fobs_cents, fobs_edges = utils.pta_freqs(dur=pta_dur * YR, num=int(nfreqs))
+    params = pspace.param_samples[samples]
+    t0 = time.perf_counter()
+    out = gwb_for_params(pspace, params, shapes=shapes, fobs_edges=fobs_edges, nreals=nreals,
+                         seeds=seed + 2 * samples.astype(np.int64) + 1, cosmo=cosmo, streams=streams, chunk=chunk,
+                         reduce=reduce, plan_workers=plan_workers, lookahead=lookahead, log_every=log_every)
+    lib = {'fobs_cents': fobs_cents, 'fobs_edges': fobs_edges, 'sample_params': params.copy(),
+           ('gwb' if reduce is None else reduce_key): out,
+           'samples_per_s': len(samples) / (time.perf_counter() - t0),
+           'param_names': np.array(pspace.param_names).astype('S'),
+           'parameter_space_class_name': np.array(pspace.name)}
+    return lib
+
+
+def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_REALS, seeds=0, cosmo=None,
+                   streams=3, chunk=256, reduce=None, plan_workers=8, lookahead=16, log_every=0):
+    """Poisson GWB h_c (n, F, R) for parameter rows ``params`` (n, D) of ``pspace`` (unvaried
+    parameters from its ``DEFAULTS``), or ``reduce`` of it per block of samples, concatenated.
+
+    Row ``i`` runs on stream ``i % streams`` (default priority): hardening norms, density and
+    scatter, evolution, number, strain and realizations are launched there, and its (F, R) result
+    stays in HBM until an asynchronous copy into a pinned host block of ``chunk`` rows, read after
+    one synchronisation per block.  With per-row ``shapes``, the scatter geometry of grids not seen
+    before is built by ``plan_workers`` host processes up to ``lookahead`` rows ahead (0: in line,
+    when a row needs it).  ``seeds``: one Philox seed per row, or a base seed ``s`` giving row
+    ``i`` the seed ``s + i``.
+    """
+    import torch
+    from . import engine, _lib, scatter as _scatter
+    from . import device as D
+    dev = D.require_device()
+    params = np.atleast_2d(np.asarray(params, dtype=float))
+    S = params.shape[0]
+    if fobs_edges is None:
+        fobs_edges = utils.pta_freqs(dur=DEF_PTA_DUR * YR, num=DEF_NUM_FBINS)[1]
+    fobs_edges = np.asarray(fobs_edges, dtype=float)
+    F, R = fobs_edges.size - 1, int(nreals)
+    seeds = np.asarray(seeds, dtype=np.int64)
+    if seeds.ndim == 0:
+        seeds = int(seeds) + np.arange(S, dtype=np.int64)
+    if seeds.shape != (S,):
+        raise ValueError(f"{seeds.shape=} must be () or ({S},)")
+    if cosmo is None:
+        from .cosmology import cosmo as _cos
+        cosmo = _cos
+    cosmo.device_tables()
+    torch.cuda.synchronize()
+    if S == 0:
+        return np.zeros((0, F, R)) if reduce is None else None
+
+    pool = _scatter.geometry_pool(int(plan_workers)) if (plan_workers and shapes is not None) else None
+    pending = {}
+    sams_ = {}
+
+    def prefetch(ii):
+        # the (host-only) SAM of row ii, and its grid's scatter geometry built by a worker thread
+        if ii >= S or ii in sams_:
+            return
+        shape = pspace.sam_shape if shapes is None else tuple(int(v) for v in shapes[ii])
+        settings = dict(pspace.DEFAULTS)
+        for name, value in zip(pspace.param_names, params[ii]):
+            settings[name] = float(value)
+        sam = pspace._init_sam(shape, settings, cosmo=cosmo)
+        sams_[ii] = (sam, settings)
+        if pool is None or sam._mmbulge._scatter_dex <= 0.0:
+            return
+        key = _scatter.plan_key(sam.mtot, sam.mrat)
+        if key in pending or _scatter.has_plan(sam.mtot, sam.mrat):
+            return
+        pending[key] = pool.submit(_scatter.build_geometry, sam.mtot, sam.mrat, 4, 1)
+
+    streams_ = [torch.cuda.Stream(device=dev) for _ in range(max(1, int(streams)))]
+    block = torch.empty((min(chunk, S), F, R), dtype=torch.float64, pin_memory=True)
+    out = np.zeros((S, F, R)) if reduce is None else []
+    t0 = time.perf_counter()
+    try:
+        for ii in range(min(lookahead, S)):
+            prefetch(ii)
+        start = 0
+        while start < S:
+            n = min(chunk, S - start)
+            for j in range(n):
+                ii = start + j
+                prefetch(ii + lookahead)
+                sam, settings = sams_.pop(ii)
+                fut = pending.pop(_scatter.plan_key(sam.mtot, sam.mrat), None)
+                if fut is not None:
+                    _scatter.put_plan(_scatter.ScatterPlan(sam.mtot, sam.mrat, geom=fut.result()), sam.mtot,
+                                      sam.mrat)
+                with torch.cuda.stream(streams_[ii % len(streams_)]):
+                    hard = pspace._init_hard(sam, settings)
+                    number, h2fdf, _ = sam._number_h2fdf_device(fobs_edges, hard)
+                    _, _, hc2, _ = engine.realize(number, h2fdf, R, 0, _lib.HD_SEL_NONE, seed=int(seeds[ii]))
+                    block[j].copy_(hc2, non_blocking=True)
+            torch.cuda.synchronize()
+            hc = np.sqrt(block[:n].numpy())
+            if reduce is None:
+                out[start:start + n] = hc
+            else:
+                out.append(np.asarray(reduce(hc)))
+            start += n
+            if log_every and (start // log_every) != ((start - n) // log_every):
+                dt = time.perf_counter() - t0
+                print(f"gwb_for_params: {start}/{S} samples, {dt:.1f} s ({start / dt:.2f} samples/s)", flush=True)
+    finally:
+        for fut in pending.values():
+            fut.cancel()
+    return out if reduce is None else np.concatenate(out, axis=0)
 
 
 def load_library(path):

@@ -16,25 +16,30 @@ both act on the m1 axis; the slice result is (W W)^T D (D indexed [m1, m2]).
 This is reproduced as is.
 
 Split of work:
-* the PLAN -- geometry only, built once per (mtot, mrat, scatter) on the host:
+* the PLAN -- geometry only, built once per (mtot, mrat) on the host:
   the Delaunay triangulation of the nodes (scipy.spatial.Delaunay / Qhull, as
   the reference's interpolator builds it), vertex adjacency in Qhull's order,
   a level schedule of the Gauss-Seidel sweep, the simplex and barycentric
   coordinates of every regular-grid point, each simplex's cross-boundary
-  derivative directions, the nearest node of every regular-grid point, the
-  Gaussian-CDF weight matrix and the bilinear weights back to the nodes;
+  derivative directions, the nearest node of every regular-grid point and the
+  bilinear weights back to the nodes; the Gaussian-CDF weights of a scatter
+  value are made per call (P = (W W)^T by an FP64 GEMM on the device);
 * the DEVICE -- everything that depends on the density: the global gradient
   estimation (Gauss-Seidel, level-scheduled, identical update sequence), the
   Clough-Tocher evaluation and bad-value fill, the FP64 GEMM with (W W)^T and
   the bilinear gather, for all redshift slices at once.
 """
+import atexit
+import collections
+
 import numpy as np
 import torch
 
 from . import _lib
 from . import device as D
 
-_PLANS = {}
+_PLANS = collections.OrderedDict()
+MAX_PLANS = 16          # geometry plans kept (least recently used dropped first)
 
 
 def _rolled_weights(log_cents, scatter):
@@ -83,97 +88,191 @@ def _simplex_g(tri):
     return g
 
 
+def _find_simplex(tri, pts, xi, margin=1e-6):
+    """``tri.find_simplex(xi)`` with the points clearly outside the convex hull (farther than
+    `margin` beyond a hull facet) set to -1 without a search.  Most regular-grid points lie
+    outside the hull of the (M, q) nodes, and Qhull's search is slowest for exactly those; the
+    points it still searches get the same answer as before."""
+    from scipy.spatial import ConvexHull
+    eq = ConvexHull(pts).equations                    # outward unit normals and offsets
+    dist = np.full(xi.shape[0], -np.inf)
+    for c in range(0, eq.shape[0], 64):
+        e = eq[c:c + 64]
+        dist = np.maximum(dist, (xi @ e[:, :2].T + e[:, 2]).max(axis=1))
+    near = dist <= margin
+    out = np.full(xi.shape[0], -1, dtype=np.int64)
+    out[near] = tri.find_simplex(xi[near])
+    return out
+
+
+def build_geometry(mtot, mrat, refine=4, kd_workers=-1):
+    """Host geometry of the scatter step for one (mtot, mrat) grid (numpy/scipy only, so it can be
+    built in a worker process).  Returns a dictionary of arrays and sizes for :class:`ScatterPlan`."""
+    from scipy.spatial import Delaunay, cKDTree
+    mtot = np.asarray(mtot, dtype=float)
+    mrat = np.asarray(mrat, dtype=float)
+    M, Q = mtot.size, mrat.size
+    m1 = mtot[:, None] / (1.0 + mrat[None, :])
+    m2 = mtot[:, None] - m1
+    pts = np.stack([np.log10(m1.ravel()), np.log10(m2.ravel())], axis=-1)
+    G = M * refine
+    lo = 0.9 * mtot[0] * mrat[0] / (1.0 + mrat[0])
+    hi = mtot[-1] * (1.0 + mrat[0]) / mrat[0]
+    mextr = np.array([min(lo, hi), max(lo, hi)])
+    mgrid = np.log10(np.logspace(*np.log10(mextr), G))
+    tri = Delaunay(pts)
+    indptr, indices = tri.vertex_neighbor_vertices
+    N = pts.shape[0]
+    # level schedule: a vertex may update once all lower-indexed neighbours have
+    level = np.zeros(N, dtype=np.int64)
+    for p in range(N):
+        nb = indices[indptr[p]:indptr[p + 1]]
+        lo_nb = nb[nb < p]
+        level[p] = (level[lo_nb].max() + 1) if lo_nb.size else 0
+    nlev = int(level.max()) + 1
+    lorder = np.argsort(level, kind='stable')
+    lptr = np.concatenate([[0], np.cumsum(np.bincount(level, minlength=nlev))])
+    # regular-grid query points xi[i*G + j] = (mgrid[i], mgrid[j])
+    X, Y = np.meshgrid(mgrid, mgrid, indexing='ij')
+    xi = np.stack([X.ravel(), Y.ravel()], axis=-1)
+    simp = _find_simplex(tri, pts, xi)
+    bary = np.zeros((xi.shape[0], 3))
+    ok = simp >= 0
+    Tm = tri.transform[simp[ok], :2, :]
+    r = tri.transform[simp[ok], 2, :]
+    c01 = np.einsum('nij,nj->ni', Tm, xi[ok] - r)
+    bary[ok, :2] = c01
+    bary[ok, 2] = 1.0 - c01.sum(axis=1)
+    nearest = cKDTree(pts).query(xi, workers=kd_workers)[1]
+    # bilinear back-interpolation (RegularGridInterpolator 'linear') at the nodes
+    ii = np.clip(np.searchsorted(mgrid, pts[:, 0]) - 1, 0, G - 2)
+    jj = np.clip(np.searchsorted(mgrid, pts[:, 1]) - 1, 0, G - 2)
+    wx = (pts[:, 0] - mgrid[ii]) / (mgrid[ii + 1] - mgrid[ii])
+    wy = (pts[:, 1] - mgrid[jj]) / (mgrid[jj + 1] - mgrid[jj])
+    back_idx = np.stack([ii * G + jj, ii * G + jj + 1, (ii + 1) * G + jj, (ii + 1) * G + jj + 1], axis=-1)
+    back_w = np.stack([(1 - wx) * (1 - wy), (1 - wx) * wy, wx * (1 - wy), wx * wy], axis=-1)
+    host = dict(
+        pts=pts, nbr_ptr=indptr.astype(np.int32), nbr_idx=indices.astype(np.int32),
+        levels_ptr=lptr.astype(np.int32), levels_idx=lorder.astype(np.int32),
+        simplices=tri.simplices.astype(np.int32), simplex_g=_simplex_g(tri), q_simplex=simp.astype(np.int32),
+        q_bary=bary, q_nearest=nearest.astype(np.int32),
+        back_idx=back_idx.astype(np.int32), back_w=back_w)
+    return dict(host=host, M=M, Q=Q, G=G, N=N, n_levels=nlev, mgrid=mgrid)
+
+
 class ScatterPlan:
-    """Geometry of the scatter step for one (mtot, mrat, scatter) triple."""
+    """Geometry of the scatter step for one (mtot, mrat) grid; the Gaussian-CDF convolution matrix
+    for a given scatter is made on demand (:meth:`P_for`).  ``geom``: the output of
+    :func:`build_geometry` if it was built elsewhere."""
 
-    def __init__(self, mtot, mrat, scatter, refine=4):
-        from scipy.spatial import Delaunay, cKDTree
-        mtot = np.asarray(mtot, dtype=float)
-        mrat = np.asarray(mrat, dtype=float)
-        M, Q = mtot.size, mrat.size
-        m1 = mtot[:, None] / (1.0 + mrat[None, :])
-        m2 = mtot[:, None] - m1
-        pts = np.stack([np.log10(m1.ravel()), np.log10(m2.ravel())], axis=-1)
-        G = M * refine
-        lo = 0.9 * mtot[0] * mrat[0] / (1.0 + mrat[0])
-        hi = mtot[-1] * (1.0 + mrat[0]) / mrat[0]
-        mextr = np.array([min(lo, hi), max(lo, hi)])
-        mgrid = np.log10(np.logspace(*np.log10(mextr), G))
-        tri = Delaunay(pts)
-        indptr, indices = tri.vertex_neighbor_vertices
-        N = pts.shape[0]
-        # level schedule: a vertex may update once all lower-indexed neighbours have
-        level = np.zeros(N, dtype=np.int64)
-        for p in range(N):
-            nb = indices[indptr[p]:indptr[p + 1]]
-            lo_nb = nb[nb < p]
-            level[p] = (level[lo_nb].max() + 1) if lo_nb.size else 0
-        nlev = int(level.max()) + 1
-        lorder = np.argsort(level, kind='stable')
-        lptr = np.concatenate([[0], np.cumsum(np.bincount(level, minlength=nlev))])
-        # regular-grid query points xi[i*G + j] = (mgrid[i], mgrid[j])
-        X, Y = np.meshgrid(mgrid, mgrid, indexing='ij')
-        xi = np.stack([X.ravel(), Y.ravel()], axis=-1)
-        simp = tri.find_simplex(xi)
-        bary = np.zeros((xi.shape[0], 3))
-        ok = simp >= 0
-        Tm = tri.transform[simp[ok], :2, :]
-        r = tri.transform[simp[ok], 2, :]
-        c01 = np.einsum('nij,nj->ni', Tm, xi[ok] - r)
-        bary[ok, :2] = c01
-        bary[ok, 2] = 1.0 - c01.sum(axis=1)
-        nearest = cKDTree(pts).query(xi)[1]
-        W = _rolled_weights(mgrid, scatter)
-        # bilinear back-interpolation (RegularGridInterpolator 'linear') at the nodes
-        ii = np.clip(np.searchsorted(mgrid, pts[:, 0]) - 1, 0, G - 2)
-        jj = np.clip(np.searchsorted(mgrid, pts[:, 1]) - 1, 0, G - 2)
-        wx = (pts[:, 0] - mgrid[ii]) / (mgrid[ii + 1] - mgrid[ii])
-        wy = (pts[:, 1] - mgrid[jj]) / (mgrid[jj + 1] - mgrid[jj])
-        back_idx = np.stack([ii * G + jj, ii * G + jj + 1, (ii + 1) * G + jj, (ii + 1) * G + jj + 1], axis=-1)
-        back_w = np.stack([(1 - wx) * (1 - wy), (1 - wx) * wy, wx * (1 - wy), wx * wy], axis=-1)
-
-        self.M, self.Q, self.G, self.N = M, Q, G, N
-        self.n_levels = nlev
-        self.host = dict(
-            pts=pts, nbr_ptr=indptr.astype(np.int32), nbr_idx=indices.astype(np.int32),
-            levels_ptr=lptr.astype(np.int32), levels_idx=lorder.astype(np.int32),
-            simplices=tri.simplices.astype(np.int32), simplex_g=_simplex_g(tri), q_simplex=simp.astype(np.int32),
-            q_bary=bary, q_nearest=nearest.astype(np.int32), wT=np.ascontiguousarray(W.T),
-            back_idx=back_idx.astype(np.int32), back_w=back_w)
-        self.tri = tri
-        self.mgrid = mgrid
-        self.W = W
+    def __init__(self, mtot, mrat, refine=4, geom=None):
+        g = build_geometry(mtot, mrat, refine) if geom is None else geom
+        self.M, self.Q, self.G, self.N = g['M'], g['Q'], g['G'], g['N']
+        self.n_levels = g['n_levels']
+        self.host = g['host']
+        self.mgrid = g['mgrid']
         self._dev = None
+        self._P = {}
 
     def device(self):
+        """Device copies of the geometry for work on the current stream.  The uploads are
+        synchronous; each use marks the tensors as used by the current stream, so that a plan
+        dropped from the cache is not recycled while another stream still reads it."""
         if self._dev is None:
             h = self.host
             d = {}
             for k, v in h.items():
                 d[k] = D.to_dev(v, dtype=torch.int32 if v.dtype == np.int32 else torch.float64)
-            # P = (W W)^T = W^T W^T on the device
-            G = self.G
-            P = D.empty((G, G))
-            _lib.call("hd_gemm_f64", D.ptr(d['wT']), D.ptr(d['wT']), D.ptr(P), G, G, G, D.stream_ptr())
-            d['P'] = P
             self._dev = d
+        cur = torch.cuda.current_stream()
+        for t in self._dev.values():
+            t.record_stream(cur)
         return self._dev
 
+    def P_for(self, scatter, keep=8):
+        """P = (W W)^T = W^T W^T for the Gaussian-CDF weights of `scatter` [dex], made on the device
+        on the current stream (the last `keep` scatter values are cached; a later user on another
+        stream waits for the product)."""
+        key = float(scatter)
+        cur = torch.cuda.current_stream()
+        ent = self._P.get(key)
+        if ent is None:
+            G = self.G
+            wT = D.to_dev(np.ascontiguousarray(_rolled_weights(self.mgrid, scatter).T))
+            P = D.empty((G, G))
+            _lib.call("hd_gemm_f64", D.ptr(wT), D.ptr(wT), D.ptr(P), G, G, G, D.stream_ptr())
+            ev = torch.cuda.Event()
+            ev.record(cur)
+            if len(self._P) >= keep:
+                self._P.pop(next(iter(self._P)))
+            self._P[key] = (P, ev)
+            return P
+        P, ev = ent
+        cur.wait_event(ev)
+        P.record_stream(cur)
+        return P
 
-def get_plan(mtot, mrat, scatter):
-    key = (np.asarray(mtot).tobytes(), np.asarray(mrat).tobytes(), float(scatter))
+
+_POOL = None
+
+
+def _pool_init():
+    # one thread per worker process: the pool supplies the parallelism
+    from threadpoolctl import threadpool_limits
+    threadpool_limits(1)
+
+
+def geometry_pool(workers):
+    """A process pool (spawned interpreters, reused across calls) for :func:`build_geometry`."""
+    global _POOL
+    import concurrent.futures
+    import multiprocessing
+    if _POOL is None or _POOL._max_workers != workers:
+        if _POOL is not None:
+            _POOL.shutdown(wait=True, cancel_futures=True)
+        _POOL = concurrent.futures.ProcessPoolExecutor(max_workers=workers,
+                                                       mp_context=multiprocessing.get_context("spawn"),
+                                                       initializer=_pool_init)
+        atexit.register(_POOL.shutdown, wait=True, cancel_futures=True)
+    return _POOL
+
+
+def plan_key(mtot, mrat):
+    return (np.asarray(mtot, dtype=float).tobytes(), np.asarray(mrat, dtype=float).tobytes())
+
+
+def has_plan(mtot, mrat):
+    return plan_key(mtot, mrat) in _PLANS
+
+
+def put_plan(plan, mtot, mrat):
+    """Insert a plan built elsewhere (e.g. ahead of time in a worker thread) into the cache."""
+    key = plan_key(mtot, mrat)
+    while len(_PLANS) >= MAX_PLANS:
+        _PLANS.popitem(last=False)
+    _PLANS[key] = plan
+
+
+def get_plan(mtot, mrat):
+    """Cached geometry plan of one (mtot, mrat) grid."""
+    key = plan_key(mtot, mrat)
     plan = _PLANS.get(key)
     if plan is None:
-        plan = ScatterPlan(mtot, mrat, scatter)
+        plan = ScatterPlan(mtot, mrat)
+        while len(_PLANS) >= MAX_PLANS:
+            _PLANS.popitem(last=False)
         _PLANS[key] = plan
+    else:
+        _PLANS.move_to_end(key)
     return plan
 
 
 def add_scatter_to_masses_device(mtot, mrat, dens, scatter, tol=1e-6, maxiter=400):
     """In place on the (M, Q, Z) device density; returns the per-slice Gauss-Seidel sweep counts (0 = no
     convergence within maxiter, as the reference's estimator reports)."""
-    plan = get_plan(mtot, mrat, scatter)
-    d = plan.device()
+    plan = get_plan(mtot, mrat)
+    d = dict(plan.device())
+    d['P'] = plan.P_for(scatter)
     M, Q, Z = dens.shape
     assert (M, Q) == (plan.M, plan.Q)
     ws = D.workspace(_lib.size("hd_scatter_workspace_size", M, Q, Z, plan.G))

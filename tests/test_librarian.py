@@ -68,3 +68,26 @@ def test_small_library_on_gpu(gpu, tmp_path):
     one = L.run_model(sam, hard, nfreqs=6, nreals=20, nloudest=3, seed=5 + 2)
     for key in ("gwb", "hc_ss", "hc_bg"):
         assert np.array_equal(one[key], lib[key][1]), key
+
+
+@pytest.mark.gpu
+def test_pipelined_gwb_library_matches_run_library(gpu):
+    """gwb_library over 3 streams, blocks of 3 samples and a 2-entry scatter-plan cache (so plans are
+    dropped while other streams may still read them) gives the same GWB, bit for bit, as run_library."""
+    print("[gwb_library]", flush=True)
+    from holodeck_amd import scatter
+    ps = L.PS_Classic_Phenom_Uniform(nsamples=7, seed=13)
+    shapes = np.array([[21, 17, 15], [25, 13, 19], [17, 21, 11], [21, 17, 15], [19, 15, 13], [23, 11, 17],
+                       [17, 21, 11]])
+    old = scatter.MAX_PLANS
+    scatter.MAX_PLANS = 2
+    try:
+        lib = L.gwb_library(ps, shapes=shapes, nfreqs=5, nreals=16, seed=9, streams=3, chunk=3)
+        tg = L.gwb_library(ps, shapes=shapes, nfreqs=5, nreals=16, seed=9, streams=2, chunk=4,
+                           reduce=lambda hc: np.median(hc, axis=-1), reduce_key='med')
+    finally:
+        scatter.MAX_PLANS = old
+    ref = L.run_library(ps, shapes=shapes, nfreqs=5, nreals=16, seed=9, singles_flag=False)
+    assert lib["gwb"].shape == (7, 5, 16) and lib["samples_per_s"] > 0
+    assert np.array_equal(lib["gwb"], ref["gwb"])
+    assert np.array_equal(tg["med"], np.median(ref["gwb"], axis=-1))

@@ -105,3 +105,43 @@ def test_mlp_training_learns_a_smooth_map(gpu):
     rms0 = np.sqrt(np.mean((truth - f(Xtr).mean(axis=0)) ** 2))
     assert rms < 0.1 * rms0, (rms, rms0)
     assert np.all(net.in_box(Xte)) and not net.in_box(np.full((1, 6), 1.5))[0]
+
+
+@pytest.mark.gpu
+def test_surrogate_on_a_sam_library_gains_over_the_plain_sam(gpu, tmp_path):
+    """What the trained network gains: on a small SAM library (17x15x13 grid, 6 frequencies, 64
+    realizations) the MLP's held-out error is a fraction of the no-model guess (the training mean),
+    it answers far faster than the plain GPU SAM, it survives save/load bit for bit, and points
+    outside the training box fall back to the plain SAM exactly."""
+    import time
+    from holodeck_amd import librarian as L
+    print("[surrogate on SAM library]", flush=True)
+    ps = L.PS_Classic_Phenom_Uniform(nsamples=640, sam_shape=(17, 15, 13), seed=21)
+    t0 = time.perf_counter()
+    lib = L.gwb_library(ps, nfreqs=6, nreals=64, seed=3, reduce=S.targets_from_gwb)
+    t_sam = (time.perf_counter() - t0) / 640
+    X, Y = lib["sample_params"], lib["targets"]
+    tr, te = np.arange(576), np.arange(576, 640)
+    net = S.MLPSurrogate(n_in=6, hidden=(128, 128, 128), n_out=18, seed=1)
+    ext = ps.extrema
+    net.fit(X[tr], Y[tr], epochs=400, batch=64, lr=2e-3, lr_min=1e-4, seed=0, x_lo=ext[:, 0], x_hi=ext[:, 1])
+    err = np.median(np.abs(net.predict(X[te]) - Y[te]))
+    err0 = np.median(np.abs(Y[tr].mean(axis=0) - Y[te]))
+    print(f"held-out median error {err:.4f} dex, mean-guess {err0:.4f} dex", flush=True)
+    assert err < 0.25 * err0, (err, err0)
+    Xq = np.repeat(X[te], 100, axis=0)
+    net.predict(Xq[:64])
+    t0 = time.perf_counter()
+    net.predict(Xq)
+    t_mlp = (time.perf_counter() - t0) / Xq.shape[0]
+    assert t_sam > 20 * t_mlp, (t_sam, t_mlp)
+    path = tmp_path / "w.npz"
+    net.save(str(path), note="x")
+    back = S.MLPSurrogate.load(str(path))
+    assert np.array_equal(back.predict(X[te]), net.predict(X[te])) and str(back.meta["note"]) == "x"
+    sw = S.SurrogateWithFallback(net, ps, nfreqs=6, nreals=64)
+    Xo = np.vstack([X[te[:4]], ext[:, 1] + 0.1])          # last row outside the box
+    Yo, rep = sw.predict(Xo, judge_frac=0.5, seed=11)
+    assert rep["n_fallback"] == 1 and rep["n_judged"] == 2 and rep["median_dex_error"] is not None
+    assert np.array_equal(Yo[-1], sw.sam_targets(Xo[-1:], seed=11)[0])
+    assert np.array_equal(Yo[:4], net.predict(Xo[:4]))
