@@ -8,6 +8,15 @@ strain, frequency-0 order, 500 Poisson realizations with the 10 loudest
 sources per frequency split from the background.  Inputs are parameters and
 1-D grids; every per-cell value is computed inside the timed region.
 
+The surrogate half of the metric: the trained MLP (weights shipped in the
+package, trained on 20k Latin-hypercube samples of the classic uniform space on
+the same grid, 30 frequencies, 500 realizations) predicts log10 h_c median,
+5th and 95th percentile spectra for 1e5 uniform parameter vectors, and the
+plain GPU SAM recomputes 1% of them as judge.  Reported: surrogate spectra/s
+with and without the judge, the judged median error in dex, the plain SAM's
+spectra/s at the surrogate's configuration (from the judge runs), and the MFMA
+roofline fraction of the inference GEMMs.
+
 Prints one JSON line (rank 0).  --gpus N>1 is rejected: this build runs on one GPU.
 """
 import argparse
@@ -24,6 +33,8 @@ sys.path.insert(0, ROOT)
 METRIC = "GWB spectra/sec on 1 MI355X: plain GPU SAM vs MLP surrogate at ≤0.05 dex err"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_hd_realize_91x81x101_f40_r500.json")
+SURROGATE_WEIGHTS = os.path.join(ROOT, "holodeck_amd", "data", "surrogate_classic_uniform_91x81x101_f30_r500.npz")
+BF16_DENSE_PEAK_TFS = 2500.0
 CALIB_BYTES = 1 << 30   # copy-kernel calibration size for the PMC passes (bytes read = bytes written)
 
 
@@ -83,6 +94,49 @@ def cpu_baseline(spec, nz_sample=11, r_sample=20):
                 stage_a_s=t_a_full, realize_s=t_b_full, measured_s=t_a + t_b)
 
 
+def surrogate_bench(n, judge_frac, seed=7):
+    """Surrogate inference on n uniform parameter vectors in the training box, the plain GPU SAM
+    judging judge_frac of them (timed pieces and errors)."""
+    import torch
+    from holodeck_amd import _lib, librarian, surrogate
+    net = surrogate.MLPSurrogate.load(SURROGATE_WEIGHTS)
+    shape = tuple(int(v) for v in net.meta['shape'])
+    F = int(net.meta['fobs_cents'].size)
+    ps = librarian.PS_Classic_Phenom_Uniform(sam_shape=shape)
+    rng = np.random.default_rng(seed)
+    X = net.x_lo + (net.x_hi - net.x_lo) * rng.random((n, net.sizes[0]))
+    net.predict(X[:1024])
+    torch.cuda.synchronize()
+    _lib.timing_enable(True)
+    t0 = time.perf_counter()
+    net.predict(X)
+    t_inf = time.perf_counter() - t0
+    gemm_ms, gemm_n = _lib.timing_read("k_mlp_gemm")
+    _lib.timing_enable(False)
+    flops = 2.0 * n * sum(a * b for a, b in zip(net.sizes[:-1], net.sizes[1:]))
+    sw = surrogate.SurrogateWithFallback(net, ps, nfreqs=F, nreals=int(net.meta['nreals']))
+    _, rep = sw.predict(X, judge_frac=judge_frac, seed=seed)
+    sam_rate = rep["n_judged"] / rep["t_judge_s"] if rep["n_judged"] else None
+    tf = flops / (gemm_ms * 1e-3) / 1e12
+    return {
+        "config": {"workload": (f"MLP {'-'.join(str(v) for v in net.sizes)} (SiLU, BF16 MFMA, FP32 accumulate), "
+                                f"trained on {int(net.meta['ntrain'])} classic-uniform samples of the {shape[0]}x"
+                                f"{shape[1]}x{shape[2]} grid, {F} frequencies, {int(net.meta['nreals'])} realizations; "
+                                f"{n} uniform inference vectors, {judge_frac:.0%} judged by the plain GPU SAM")},
+        "spectra_per_s": n / (t_inf + rep["t_judge_s"]),
+        "spectra_per_s_inference_only": n / t_inf,
+        "plain_sam_spectra_per_s_same_config": sam_rate,
+        "x_vs_plain_sam": (n / (t_inf + rep["t_judge_s"])) / sam_rate if sam_rate else None,
+        "x_vs_plain_sam_inference_only": (n / t_inf) / sam_rate if sam_rate else None,
+        "judged": rep["n_judged"], "judged_median_dex_error": rep["median_dex_error"],
+        "judged_p95_dex_error": rep["p95_dex_error"],
+        "holdout_median_dex_error": float(net.meta['holdout_median_dex_error']),
+        "roofline": {"bound": "mfma", "kernel": "k_mlp_gemm (inference forward)", "achieved": tf,
+                     "peak": BF16_DENSE_PEAK_TFS, "unit": "TFLOP/s", "frac": tf / BF16_DENSE_PEAK_TFS,
+                     "flops": flops, "gemm_ms": gemm_ms, "launches": gemm_n},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +146,9 @@ def main():
     ap.add_argument("--nreals", type=int, default=500)
     ap.add_argument("--streams", type=int, default=3,
                     help="independent spectra in flight on this many HIP streams (consecutive steps alternate)")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-n", type=int, default=100000)
+    ap.add_argument("--judge-frac", type=float, default=0.01)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy a 1 GiB device buffer once (reference kernel for PMC byte "
                          "counters)")
@@ -178,6 +235,8 @@ def main():
                      "traffic": traffic, "alg_bytes_per_launch": alg_bytes, "launch_ms": t_real * 1e3,
                      "launches": real_n, "k_gz_draw_ms": draw_ms / max(draw_n, 1)},
     }
+    if not args.no_surrogate and os.path.exists(SURROGATE_WEIGHTS):
+        line["surrogate"] = surrogate_bench(args.surrogate_n, args.judge_frac)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec)
     print(json.dumps(line), flush=True)

@@ -203,6 +203,7 @@ extern "C" int hd_mlp_gemm_bf16(const void* A, int64_t sam, int64_t sak, const v
     bf16* ob = (bf16*)out_bf16;
     hipStream_t st = (hipStream_t)stream;
     dim3 grid((unsigned)cdiv(N, NT), (unsigned)cdiv(M, MT));
+    KTimer timer("k_mlp_gemm", st);
     switch (epi) {
         case HD_MLP_EPI_STORE:
             if (!out_f32) return set_error(HD_ERR_ARG, "hd_mlp_gemm_bf16: STORE needs out_f32");
