@@ -55,6 +55,12 @@ SIGNATURES = {
     "hd_mlp_colsum_bf16": (_I32, [_P, _I32, _I32, _P, _P]),
     "hd_mlp_mse": (_I32, [_P, _P, _I64, _P, _P, _P]),
     "hd_mlp_adam": (_I32, [_I64, _P, _P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float, _I32, _P, _P]),
+    "hd_eccen_workspace_size": (_SZ, [_I32, _I32, _I32]),
+    "hd_gwb_eccen": (_I32, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _P, _P, _SZ, _P]),
+    "hd_eccen_number": (_I32, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _I32, _P, _P, _P,
+                               _SZ, _P]),
+    "hd_bessel_jpair": (_I32, [_P, _P, _I64, _P, _P]),
+    "hd_gw_freq_dist": (_I32, [_P, _P, _I64, _P, _P]),
     "hd_f32_to_bf16": (_I32, [_P, _I64, _P, _P]),
 }
 

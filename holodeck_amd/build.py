@@ -13,7 +13,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libholohip.so")
 
 SOURCES = ["capi.hip", "cosmo.hip", "density.hip", "scatter.hip", "evolve.hip", "strain.hip", "sort.hip",
-           "realize.hip", "realize_gap.hip", "batch.hip", "mlp.hip"]
+           "realize.hip", "realize_gap.hip", "batch.hip", "mlp.hip", "eccen.hip"]
 
 ARCH = os.environ.get("HOLOHIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -222,6 +222,36 @@ int hd_mlp_adam(int64_t n, float *w, const float *g, float *m, float *v, float l
                 int step, void *w_bf16, void *stream);
 int hd_f32_to_bf16(const float *a, int64_t n, void *b, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Eccentric SAM GWB over one eccentricity evolution e(a) shared by all binaries.
+ * Inputs as the reference's Cython signatures: ndens (M,Q,Z) at the grid nodes
+ * [Mpc^-3], log10 of the total-mass edges [g], mass-ratio and redshift edges, comoving
+ * distances (Z) [Mpc], observer GW frequencies (F) [1/s], the evolution's separations
+ * (E) [cm] (decreasing) and eccentricities (E).
+ * ---------------------------------------------------------------------- */
+size_t hd_eccen_workspace_size(int M, int Z, int E);
+/* <- cyutils.sam_calc_gwb_single_eccen (holodeck/cyutils.pyx:361-597): gwb (F, nharms),
+ * the characteristic strain squared per frequency and harmonic n = 1..nharms. */
+int hd_gwb_eccen(const double *ndens, const double *mtot_log10, int M, const double *mrat, int Q, const double *redz,
+                 const double *dcom, int Z, const double *gwfobs, int F, const double *sepa_evo,
+                 const double *eccen_evo, int E, int nharms, double *gwb, void *workspace, size_t ws_bytes,
+                 void *stream);
+/* <- the per-(node, frequency, harmonic) terms of cyutils.sam_calc_gwb_single_eccen_discrete
+ * (holodeck/cyutils.pyx:766-838) for harmonics h0+1 .. h0+hc: number (M*Q*Z, F*hc) is the
+ * Poisson mean `number_term` and hval (M*Q*Z, F*hc) the strain factor `hterm / weight` of each
+ * drawn count; column aa*hc + (n-1-h0).  Zero where the reference draws nothing or adds only
+ * zeros.  Requires M*Z <= 65535.  The realizations are hd_realize with HD_SEL_NONE over these
+ * columns. */
+int hd_eccen_number(const double *ndens, const double *mtot_log10, int M, const double *mrat, int Q,
+                    const double *redz, const double *dcom, int Z, const double *gwfobs, int F, const double *sepa_evo,
+                    const double *eccen_evo, int E, int h0, int hc, double *number, double *hval, void *workspace,
+                    size_t ws_bytes, void *stream);
+/* <- the Bessel pair J_{n-2}(ne), J_{n-1}(ne) that cyutils.pyx:181-182 takes from
+ * scipy.special.jv: out[2i], out[2i+1] = J_lo[i](x[i]), J_{lo[i]+1}(x[i]) for lo >= -1, x > 0. */
+int hd_bessel_jpair(const int *lo, const double *x, int64_t n, double *out, void *stream);
+/* <- cyutils.gw_freq_dist_func__scalar_scalar (holodeck/cyutils.pyx:145-199): g(n, e) elementwise. */
+int hd_gw_freq_dist(const int *nn, const double *ee, int64_t n, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
