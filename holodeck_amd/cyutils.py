@@ -125,3 +125,24 @@ def sam_poisson_gwb(dist, hc2, nreals, normal_threshold=1e10, *, seed=None, coun
     (_, _, hc2bg, _), _ = _run(dist, hc2, nreals, 0, _lib.HD_SEL_NONE, normal_threshold=normal_threshold, seed=seed,
                                counts=counts)
     return D.to_host(hc2bg)
+
+
+def _eccen_dev(ndens, mtot_log10, mrat, redz, dcom, gwfobs, sepa_evo, eccen_evo):
+    return (D.to_dev(np.asarray(ndens, dtype=float)).contiguous(), D.to_dev(mtot_log10), D.to_dev(mrat),
+            D.to_dev(redz), D.to_dev(dcom), D.to_dev(np.atleast_1d(gwfobs)), D.to_dev(sepa_evo), D.to_dev(eccen_evo))
+
+
+def sam_calc_gwb_single_eccen(ndens, mtot_log10, mrat, redz, dcom, gwfobs, sepa_evo, eccen_evo, nharms=100):
+    """-> gwb (F, nharms): characteristic strain squared per frequency and harmonic
+    (cyutils.pyx:361-597).  ndens (M,Q,Z) at the grid nodes [Mpc^-3], dcom [Mpc]."""
+    args = _eccen_dev(ndens, mtot_log10, mrat, redz, dcom, gwfobs, sepa_evo, eccen_evo)
+    return D.to_host(engine.gwb_eccen(*args, int(nharms)))
+
+
+def sam_calc_gwb_single_eccen_discrete(ndens, mtot_log10, mrat, redz, dcom, gwfobs, sepa_evo, eccen_evo, nharms,
+                                       nreals, *, seed=None, harm_chunk=None):
+    """-> gwb (F, nharms, nreals) with Poisson counts per (node, frequency, harmonic, realization)
+    (cyutils.pyx:600-851)."""
+    args = _eccen_dev(ndens, mtot_log10, mrat, redz, dcom, gwfobs, sepa_evo, eccen_evo)
+    return D.to_host(engine.gwb_eccen_discrete(*args, int(nharms), int(nreals), seed=_seed(seed),
+                                               harm_chunk=harm_chunk))

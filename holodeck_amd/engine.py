@@ -161,3 +161,68 @@ def realize(number2d, h2fdf2d, nreals, nloud, mode, order=None, counts=None, fla
         hc2ss = hc2ss[..., :0]
         ssidx = ssidx[..., :0]
     return hc2ss, ssidx, hc2bg, bgpar
+
+
+# ---- eccentric SAM (cyutils.pyx:361-851) ----
+ECCEN_WS_COLUMNS_BYTES = 8 << 30    # device bytes budget of one harmonic chunk of the discrete variant
+
+
+def _eccen_args(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d):
+    M, Q, Z = ndens.shape
+    for t, n in ((mtot_log10_d, M), (mrat_d, Q), (redz_d, Z), (dcom_d, Z)):
+        if t.numel() != n:
+            raise ValueError("grid arrays must match the (M, Q, Z) density")
+    if sepa_d.numel() != eccen_d.numel() or sepa_d.numel() < 2:
+        raise ValueError("sepa_evo and eccen_evo must have the same length >= 2")
+    return M, Q, Z, gwfobs_d.numel(), sepa_d.numel()
+
+
+def gwb_eccen(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d, nharms):
+    """(F, nharms) characteristic strain squared per frequency and harmonic (device)."""
+    M, Q, Z, F, E = _eccen_args(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d)
+    ws = D.workspace(_lib.size("hd_eccen_workspace_size", M, Z, E))
+    out = D.empty((F, int(nharms)))
+    with _stage("eccen"):
+        _lib.call("hd_gwb_eccen", D.ptr(ndens), D.ptr(mtot_log10_d), M, D.ptr(mrat_d), Q, D.ptr(redz_d),
+                  D.ptr(dcom_d), Z, D.ptr(gwfobs_d), F, D.ptr(sepa_d), D.ptr(eccen_d), E, int(nharms), D.ptr(out),
+                  D.ptr(ws), ws.numel(), D.stream_ptr())
+    return out
+
+
+def eccen_number(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d, h0, hc):
+    """Poisson means and strain factors (M*Q*Z, F*hc) of harmonics h0+1 .. h0+hc (device)."""
+    M, Q, Z, F, E = _eccen_args(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d)
+    ws = D.workspace(_lib.size("hd_eccen_workspace_size", M, Z, E))
+    lam = D.empty((M * Q * Z, F * int(hc)))
+    hv = D.empty((M * Q * Z, F * int(hc)))
+    with _stage("eccen_number"):
+        _lib.call("hd_eccen_number", D.ptr(ndens), D.ptr(mtot_log10_d), M, D.ptr(mrat_d), Q, D.ptr(redz_d),
+                  D.ptr(dcom_d), Z, D.ptr(gwfobs_d), F, D.ptr(sepa_d), D.ptr(eccen_d), E, int(h0), int(hc),
+                  D.ptr(lam), D.ptr(hv), D.ptr(ws), ws.numel(), D.stream_ptr())
+    return lam, hv
+
+
+def eccen_harm_chunk(B, F, nharms, nreals):
+    """Harmonics per chunk of the discrete variant: Poisson means, strain factors and the
+    realization workspace of one chunk within ECCEN_WS_COLUMNS_BYTES."""
+    per_col = 16 * B + _lib.size("hd_realize_workspace_size", B, F, int(nreals), 0, 0) / max(F, 1)
+    return int(max(1, min(nharms, ECCEN_WS_COLUMNS_BYTES // (per_col * F))))
+
+
+def gwb_eccen_discrete(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d, nharms, nreals, seed=0,
+                       harm_chunk=None, normal_threshold=NORMAL_THRESHOLD):
+    """(F, nharms, R) characteristic strain squared with a Poisson count per (node, frequency,
+    harmonic, realization) (device).  Harmonics run in chunks; chunk c draws with the Philox seed
+    seed + (c+1) * 0x9E3779B97F4A7C15, so a result depends on (seed, harm_chunk)."""
+    M, Q, Z, F, E = _eccen_args(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d)
+    H, R = int(nharms), int(nreals)
+    hc = int(harm_chunk) if harm_chunk else eccen_harm_chunk(M * Q * Z, F, H, R)
+    out = D.empty((F, H, R))
+    for c, h0 in enumerate(range(0, H, hc)):
+        n = min(hc, H - h0)
+        lam, hv = eccen_number(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d, h0, n)
+        s = (int(seed) + (c + 1) * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        _, _, hc2, _ = realize(lam, hv, R, 0, _lib.HD_SEL_NONE, seed=s, normal_threshold=normal_threshold)
+        out[:, h0:h0 + n, :] = hc2.view(F, n, R)
+        del lam, hv, hc2
+    return out

@@ -40,3 +40,32 @@ def _gws_from_number_grid_integrated_redz(edges, redz, number, realize, sum=True
         from .cyutils import sam_poisson_gwb
         return np.sqrt(sam_poisson_gwb(number, hc2, realize, seed=seed))
     raise ValueError("`realize` ({}) must be one of {{True, False, integer}}!".format(realize))
+
+
+def _eccen_inputs(gwfobs, sam, sepa_evo, eccen_evo):
+    """Device inputs of the eccentric kernels for a SAM (gravwaves.py:936-941, 967-972)."""
+    from .constants import MPC
+    ndens, _, _ = sam._density_device()
+    sam._check_scatter()
+    dcom = sam.cosmo.comoving_distance(sam.redz) / MPC
+    return (ndens, D.to_dev(np.log10(sam.mtot)), D.to_dev(sam.mrat), D.to_dev(sam.redz), D.to_dev(dcom),
+            D.to_dev(np.atleast_1d(np.asarray(gwfobs, dtype=float))), D.to_dev(np.asarray(sepa_evo, dtype=float)),
+            D.to_dev(np.asarray(eccen_evo, dtype=float)))
+
+
+def sam_calc_gwb_single_eccen(gwfobs, sam, sepa_evo, eccen_evo, nharms=100):
+    """(F, nharms) GWB strain squared per harmonic of an eccentric SAM (gravwaves.py:933-941);
+    ``np.sqrt(gwb.sum(axis=1))`` is h_c(f)."""
+    return D.to_host(engine.gwb_eccen(*_eccen_inputs(gwfobs, sam, sepa_evo, eccen_evo), int(nharms)))
+
+
+def sam_calc_gwb_single_eccen_discrete(gwfobs, sam, sepa_evo, eccen_evo, nharms=100, nreals=None, *, seed=None,
+                                       harm_chunk=None):
+    """(F, nharms, nreals) -- or (F, nharms) for nreals=None -- Poisson-sampled GWB strain squared
+    per harmonic of an eccentric SAM (gravwaves.py:944-981)."""
+    from .cyutils import _seed
+    squeeze = nreals is None
+    R = 1 if squeeze else int(nreals)
+    gwb = D.to_host(engine.gwb_eccen_discrete(*_eccen_inputs(gwfobs, sam, sepa_evo, eccen_evo), int(nharms), R,
+                                              seed=_seed(seed), harm_chunk=harm_chunk))
+    return gwb.squeeze() if squeeze else gwb

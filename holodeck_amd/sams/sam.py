@@ -235,3 +235,37 @@ class Semi_Analytic_Model:
 
     def gwb_ideal(self, fobs_gw, sum=True, redz_prime=None):
         raise NotImplementedError("gwb_ideal is not part of the device path")
+
+
+def evolve_eccen_uniform_single(sam, eccen_init, sepa_init, nsteps):
+    """Eccentricity along log-spaced separations from ``sepa_init`` [cm] down to three
+    Schwarzschild radii of the lightest binary, for GW-driven evolution (sam.py:1235-1288).
+
+    Classic fourth-order Runge-Kutta in separation of de/da = 1 / (da/de) (Peters 1964,
+    Eq. 5.7), eccentricity floored at zero after each step.  Host code: E sequential steps.
+    Returns (sepa (E,), eccen (E,)).
+    """
+    from ..constants import SCHW
+    if not (0.0 <= eccen_init < 1.0):
+        raise ValueError(f"eccen_init must be in [0, 1), got {eccen_init}")
+
+    def de_da(a, e):
+        e2 = e * e
+        num = 1.0 + (73.0 / 24.0) * e2 + (37.0 / 96.0) * e2 * e2
+        den = (1.0 - e2) * (1.0 + (121.0 / 304.0) * e2)
+        with np.errstate(divide='ignore'):
+            return 1.0 / ((12.0 / 19.0) * (a / e) * (num / den)) if e != 0.0 else 0.0
+
+    a_min = (SCHW * np.asarray(sam.mtot) * 3).min()
+    sepa = np.logspace(np.log10(sepa_init), np.log10(a_min), nsteps)
+    eccen = np.zeros(nsteps)
+    eccen[0] = eccen_init
+    for s in range(1, nsteps):
+        a0, e0 = sepa[s - 1], eccen[s - 1]
+        h = sepa[s] - a0
+        k1 = h * de_da(a0, e0)
+        k2 = h * de_da(a0 + h / 2.0, e0 + k1 / 2.0)
+        k3 = h * de_da(a0 + h / 2.0, e0 + k2 / 2.0)
+        k4 = h * de_da(a0 + h, e0 + k3)
+        eccen[s] = max(e0 + (k1 + 2 * k2 + 2 * k3 + k4) / 6.0, 0.0)
+    return sepa, eccen

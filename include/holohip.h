@@ -247,7 +247,8 @@ int hd_eccen_number(const double *ndens, const double *mtot_log10, int M, const 
                     const double *eccen_evo, int E, int h0, int hc, double *number, double *hval, void *workspace,
                     size_t ws_bytes, void *stream);
 /* <- the Bessel pair J_{n-2}(ne), J_{n-1}(ne) that cyutils.pyx:181-182 takes from
- * scipy.special.jv: out[2i], out[2i+1] = J_lo[i](x[i]), J_{lo[i]+1}(x[i]) for lo >= -1, x > 0. */
+ * scipy.special.jv: out[2i], out[2i+1] = J_lo[i](x[i]), J_{lo[i]+1}(x[i]) for -1 <= lo <= 100000,
+ * 0 < x <= 1e5 (NaN outside).  nharms and h0+hc above are limited to 100000 as well. */
 int hd_bessel_jpair(const int *lo, const double *x, int64_t n, double *out, void *stream);
 /* <- cyutils.gw_freq_dist_func__scalar_scalar (holodeck/cyutils.pyx:145-199): g(n, e) elementwise. */
 int hd_gw_freq_dist(const int *nn, const double *ee, int64_t n, double *out, void *stream);
