@@ -15,7 +15,9 @@ the same grid, 30 frequencies, 500 realizations) predicts log10 h_c median,
 plain GPU SAM recomputes 1% of them as judge.  Reported: surrogate spectra/s
 with and without the judge, the judged median error in dex, the plain SAM's
 spectra/s at the surrogate's configuration (from the judge runs), and the MFMA
-roofline fraction of the inference GEMMs.
+roofline fraction of the inference GEMMs.  An eccentric-SAM line (61x51x61 grid,
+40 frequencies, 100 harmonics; deterministic and 200-realization discrete
+spectra/s) follows.
 
 Prints one JSON line (rank 0).  --gpus N>1 is rejected: this build runs on one GPU.
 """
@@ -137,6 +139,38 @@ def surrogate_bench(n, judge_frac, seed=7):
     }
 
 
+def eccentric_bench(e0=0.5, nreals=200, nharms=100):
+    """Eccentric SAM on the 61x51x61 grid, 40 frequencies i/T, 100 harmonics: the deterministic
+    harmonic sum and the discrete variant with nreals Poisson realizations, one spectrum each."""
+    import torch
+    from holodeck_amd import engine, gravwaves
+    from holodeck_amd.sams.sam import Semi_Analytic_Model, evolve_eccen_uniform_single
+    from holodeck_amd.constants import PC, YR
+    sam = Semi_Analytic_Model(shape=(61, 51, 61))
+    fobs = np.arange(1, 41) / (16.03 * YR)
+    sepa, ecc = evolve_eccen_uniform_single(sam, e0, 1.0 * PC, 123)
+    ins = gravwaves._eccen_inputs(fobs, sam, sepa, ecc)
+    engine.gwb_eccen(*ins, nharms)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        det = engine.gwb_eccen(*ins, nharms)
+    torch.cuda.synchronize()
+    t_det = (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    disc = engine.gwb_eccen_discrete(*ins, nharms, nreals, seed=1)
+    torch.cuda.synchronize()
+    t_disc = time.perf_counter() - t0
+    d = det.sum(dim=1)
+    mc = disc.sum(dim=1).mean(dim=1)
+    return {"config": {"workload": (f"eccentric SAM (default components, KH2013 with 0.3 dex scatter), 61x51x61 grid, "
+                                    f"40 frequencies i/T (T=16.03 yr), {nharms} harmonics, e0={e0} at 1 pc, 123 "
+                                    f"evolution steps")},
+            "deterministic_spectra_per_s": 1.0 / t_det, "discrete_spectra_per_s": 1.0 / t_disc,
+            "discrete_nreals": nreals,
+            "discrete_mean_over_deterministic": [float(v) for v in (mc / d).cpu().numpy()[:4]]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +183,7 @@ def main():
     ap.add_argument("--roofline-steps", type=int, default=3,
                     help="single-stream steps after the timed region that time the realization launch")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-eccentric", action="store_true")
     ap.add_argument("--surrogate-n", type=int, default=100000)
     ap.add_argument("--judge-frac", type=float, default=0.01)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -249,6 +284,8 @@ def main():
     }
     if not args.no_surrogate and os.path.exists(SURROGATE_WEIGHTS):
         line["surrogate"] = surrogate_bench(args.surrogate_n, args.judge_frac)
+    if not args.no_eccentric:
+        line["eccentric"] = eccentric_bench()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec)
     print(json.dumps(line), flush=True)

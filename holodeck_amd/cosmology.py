@@ -19,6 +19,7 @@ default Tcmb0=0.
 import numpy as np
 
 from . import _lib
+from .device import torch
 from . import device as D
 from .constants import MPC, SPLC, GYR
 
@@ -69,6 +70,8 @@ class Cosmology:
         _lib.call("hd_cosmo_eval", self.Om0, self.h, D.ptr(dc_tab), D.ptr(dc_der), n, D.ptr(gz), self.size,
                   D.ptr(gage), D.ptr(gdc), None, st)
         tabs = dict(dc_tab=dc_tab, dc_der=dc_der, n_tab=n, grid_z=gz, grid_age=gage, grid_dcom=gdc)
+        # built once per device and then read from any stream: finish the build here
+        torch.cuda.current_stream().synchronize()
         self._dev[key] = tabs
         return tabs
 

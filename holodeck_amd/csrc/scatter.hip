@@ -325,6 +325,24 @@ extern "C" int hd_gemm_f64(const double* A, const double* B, double* C, int m, i
     return check_launch("k_gemm_f64");
 }
 
+namespace hd {
+// wT[k, j] = W[j, k] = dm[k - j + G - 1]: the transposed Gaussian-CDF weight matrix of
+// utils._get_rolled_weights (utils.py:464-488) from its 2G-1 distinct values
+__global__ void k_rolled_weights_T(const double* __restrict__ dm, int G, double* __restrict__ wT) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)G * G) return;
+    const int k = (int)(t / G), j = (int)(t % G);
+    wT[t] = dm[k - j + G - 1];
+}
+}  // namespace hd
+
+extern "C" int hd_scatter_weights_T(const double* dm, int G, double* wT, void* stream) {
+    if (!dm || !wT || G < 1) return set_error(HD_ERR_ARG, "hd_scatter_weights_T: bad arguments");
+    hipLaunchKernelGGL(k_rolled_weights_T, dim3((unsigned)cdiv((int64_t)G * G, 256)), dim3(256), 0,
+                       (hipStream_t)stream, dm, G, wT);
+    return check_launch("k_rolled_weights_T");
+}
+
 extern "C" size_t hd_scatter_workspace_size(int M, int Q, int Z, int G) {
     size_t N = (size_t)M * Q;
     const size_t ecap = 6 * N;

@@ -17,6 +17,7 @@ several HIP streams, with the scatter geometry of new grids built ahead on host
 threads.
 """
 import abc
+import collections
 import time
 
 import numpy as np
@@ -388,7 +389,9 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
     Row ``i`` runs on stream ``i % streams`` (default priority): hardening norms, density and
     scatter, evolution, number, strain and realizations are launched there, and its (F, R) result
     stays in HBM until an asynchronous copy into a pinned host block of ``chunk`` rows, read after
-    one synchronisation per block.  With per-row ``shapes``, the scatter geometry of grids not seen
+    one synchronisation per block; the host runs at most 2 * ``streams`` rows ahead of the device
+    (an event per row), which bounds the pinned staging memory of the uploads.  With per-row
+    ``shapes``, the scatter geometry of grids not seen
     before is built by ``plan_workers`` host processes up to ``lookahead`` rows ahead (0: in line,
     when a row needs it).  ``seeds``: one Philox seed per row, or a base seed ``s`` giving row
     ``i`` the seed ``s + i``.
@@ -440,6 +443,8 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
     streams_ = [torch.cuda.Stream(device=dev) for _ in range(max(1, int(streams)))]
     block = torch.empty((min(chunk, S), F, R), dtype=torch.float64, pin_memory=True)
     out = np.zeros((S, F, R)) if reduce is None else []
+    inflight = collections.deque()      # one event per launched row: the host stays <= depth rows ahead
+    depth = 2 * len(streams_)
     t0 = time.perf_counter()
     try:
         for ii in range(min(lookahead, S)):
@@ -460,7 +465,13 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
                     number, h2fdf, _ = sam._number_h2fdf_device(fobs_edges, hard)
                     _, _, hc2, _ = engine.realize(number, h2fdf, R, 0, _lib.HD_SEL_NONE, seed=int(seeds[ii]))
                     block[j].copy_(hc2, non_blocking=True)
+                    ev = torch.cuda.Event()
+                    ev.record()
+                inflight.append(ev)
+                if len(inflight) > depth:
+                    inflight.popleft().synchronize()
             torch.cuda.synchronize()
+            inflight.clear()
             hc = np.sqrt(block[:n].numpy())
             if reduce is None:
                 out[start:start + n] = hc

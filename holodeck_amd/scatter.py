@@ -42,9 +42,10 @@ _PLANS = collections.OrderedDict()
 MAX_PLANS = 16          # geometry plans kept (least recently used dropped first)
 
 
-def _rolled_weights(log_cents, scatter):
-    """utils._get_rolled_weights (utils.py:464-488) for a Normal(0, scatter) CDF."""
-    from scipy.stats import norm
+def _rolled_dm(log_cents, scatter):
+    """The 2N-1 distinct weights of utils._get_rolled_weights (utils.py:464-488) for a
+    Normal(0, scatter) CDF: row j of the matrix holds dm[k - j + N - 1] for bins k."""
+    from scipy.special import ndtr
     num = log_cents.size
     dx = np.diff(log_cents)
     if not np.allclose(dx, dx[0]):
@@ -52,8 +53,13 @@ def _rolled_weights(log_cents, scatter):
     dx = dx[0]
     dx = dx / 2.0 + np.arange(num) * dx
     dx = np.concatenate([-dx[::-1], dx])
-    dm = np.diff(norm(loc=0.0, scale=scatter).cdf(dx))          # (2N-1,)
-    # row j holds the weights from bin j to bins k: dm[k - j + N - 1]
+    return np.diff(ndtr(dx / scatter))                          # (2N-1,)
+
+
+def _rolled_weights(log_cents, scatter):
+    """utils._get_rolled_weights (utils.py:464-488) as a host matrix."""
+    dm = _rolled_dm(log_cents, scatter)
+    num = log_cents.size
     j = np.arange(num)[:, None]
     k = np.arange(num)[None, :]
     return dm[k - j + num - 1]
@@ -172,19 +178,25 @@ class ScatterPlan:
         self.host = g['host']
         self.mgrid = g['mgrid']
         self._dev = None
+        self._ready = None
         self._P = {}
 
     def device(self):
-        """Device copies of the geometry for work on the current stream.  The uploads are
-        synchronous; each use marks the tensors as used by the current stream, so that a plan
-        dropped from the cache is not recycled while another stream still reads it."""
+        """Device copies of the geometry for work on the current stream.  A stream other than the
+        uploading one waits for the uploads (an event); each use marks the tensors as used by the
+        current stream, so that a plan dropped from the cache is not recycled while another
+        stream still reads it."""
+        cur = torch.cuda.current_stream()
         if self._dev is None:
             h = self.host
             d = {}
             for k, v in h.items():
                 d[k] = D.to_dev(v, dtype=torch.int32 if v.dtype == np.int32 else torch.float64)
             self._dev = d
-        cur = torch.cuda.current_stream()
+            self._ready = torch.cuda.Event()
+            self._ready.record(cur)
+        else:
+            cur.wait_event(self._ready)
         for t in self._dev.values():
             t.record_stream(cur)
         return self._dev
@@ -198,7 +210,9 @@ class ScatterPlan:
         ent = self._P.get(key)
         if ent is None:
             G = self.G
-            wT = D.to_dev(np.ascontiguousarray(_rolled_weights(self.mgrid, scatter).T))
+            dm = D.to_dev(_rolled_dm(self.mgrid, scatter))
+            wT = D.empty((G, G))
+            _lib.call("hd_scatter_weights_T", D.ptr(dm), G, D.ptr(wT), D.stream_ptr())
             P = D.empty((G, G))
             _lib.call("hd_gemm_f64", D.ptr(wT), D.ptr(wT), D.ptr(P), G, G, G, D.stream_ptr())
             ev = torch.cuda.Event()

@@ -98,6 +98,9 @@ int hd_scatter(double *dens, int M, int Q, int Z, int G, const double *pts, cons
 /* C (m,n) = A (m,k) B (k,n), row-major FP64 (the scatter's convolution GEMM;
  * the reference evaluates it with numpy.einsum, utils.py:455-461). */
 int hd_gemm_f64(const double *A, const double *B, double *C, int m, int n, int k, void *stream);
+/* <- the weight matrix of utils._get_rolled_weights (holodeck/utils.py:464-488), transposed:
+ * wT (G,G) with wT[k,j] = dm[k - j + G - 1] for the 2G-1 Gaussian-CDF differences dm. */
+int hd_scatter_weights_T(const double *dm, int G, double *wT, void *stream);
 
 /* ------------------------------------------------------------------------
  * Binary evolution  <- holodeck/sams/sam_cyutils.pyx
