@@ -567,15 +567,25 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                 c1 = min(c1, c + GZ_BATCH_REC);
                 if (tid == 0) s_next = c;
                 __syncthreads();   // the previous batch's draws are done with tab / scum
-                // waves take the batch's records one at a time (windows differ in length)
-                for (;;) {
+                // waves take the batch's records one at a time (windows differ in length); the
+                // next record's header is loaded while the current one's table is built
+                {
                     int i = 0;
                     if (lane == 0) i = atomicAdd(&s_next, 1);
                     i = __shfl(i, 0, 64);
-                    if (i >= c1) break;
-                    const DRec rec = A.drec[rreg + i];
-                    if (lane == 0) brec[i - c] = BRec{rec.h, rec.word, rec.klo, rec.n, rec.off - base, 0};
-                    if (rec.n > 0) gz_build(tab + (rec.off - base), rec.lam, rec.klo, rec.n, rec.p0, lane);
+                    DRec rec;
+                    if (i < c1) rec = A.drec[rreg + i];
+                    while (i < c1) {
+                        int inext = 0;
+                        if (lane == 0) inext = atomicAdd(&s_next, 1);
+                        inext = __shfl(inext, 0, 64);
+                        DRec nrec;
+                        if (inext < c1) nrec = A.drec[rreg + inext];
+                        if (lane == 0) brec[i - c] = BRec{rec.h, rec.word, rec.klo, rec.n, rec.off - base, 0};
+                        if (rec.n > 0) gz_build(tab + (rec.off - base), rec.lam, rec.klo, rec.n, rec.p0, lane);
+                        i = inext;
+                        rec = nrec;
+                    }
                 }
                 if (c1 == nrec && ns > 0)
                     for (int j = tid; j < ns; j += bs) scum[j] = A.s_cum[reg + j];
