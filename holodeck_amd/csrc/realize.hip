@@ -7,12 +7,12 @@
 //
 // Per (frequency f, realization r) the reference draws a count n_b for every
 // bin b, fills L loud slots from bins in a fixed priority order (with
-// multiplicity) and sums n_b*h_b over the rest.  Here:
-//   * the count is either injected ((B,F,R) array: the exact parity path) or
-//     drawn from a counter-based Philox stream keyed by (seed, b, f, r);
+// multiplicity) and sums n_b*h_b over the rest.  This file takes the counts as
+// given -- an injected (B,F,R) array, the exact parity path -- and holds the
+// merge shared with the drawn-count kernels of realize_gap.hip:
 //   * a workgroup handles one frequency and one chunk of bins, one thread per
-//     realization; bin data (lambda, tail probabilities, h, priority key) are
-//     staged through LDS and shared by all realizations;
+//     realization; bin data (h, priority key, parameters) are staged through
+//     LDS and shared by all realizations;
 //   * each thread keeps the L best instances of its chunk in registers
 //     (priority = order rank for the sorted walk, (h desc, bin asc) for argmax)
 //     and the FP64 totals sum(n h), sum(n h p_k);
@@ -33,15 +33,12 @@ constexpr int RZ_SUB = 256;       // bins staged in LDS at a time
 constexpr uint64_t KEY_NONE = ~0ull;
 
 struct RealizeArgs {
-    const double* number;    // (B,F) or null for injected counts
-    const double* counts;    // (B,F,R) injected counts or null
+    const double* counts;    // (B,F,R) injected counts
     const double* h2fdf;     // (B,F)
     const int32_t* rank;     // (B) position in order (SORTED) or null
     const double* par;       // (npar,B,F) or null
     int64_t B;
     int F, R, L, mode, flags, npar, nchunks;
-    double thresh;
-    uint64_t seed;
     // partials
     double* p_total;      // (F, nchunks, R)
     double* p_ptot;       // (npar, F, nchunks, R)
@@ -74,11 +71,7 @@ __device__ __forceinline__ void list_insert(uint64_t (&lk)[LMAX], int32_t (&lb)[
     }
 }
 
-template <bool INJECTED>
 __global__ void __launch_bounds__(512) k_realize_chunk(RealizeArgs A) {
-    __shared__ double s_lam[RZ_SUB];
-    __shared__ double s_q1[RZ_SUB];
-    __shared__ double s_el[RZ_SUB];
     __shared__ double s_h[RZ_SUB];
     __shared__ uint64_t s_key[RZ_SUB];
     __shared__ double s_par[NPAR_MAX][RZ_SUB];
@@ -114,12 +107,6 @@ __global__ void __launch_bounds__(512) k_realize_chunk(RealizeArgs A) {
                 int64_t b = s0 + c;
                 double h = A.h2fdf[b * F + f];
                 s_h[c] = h;
-                if (!INJECTED) {
-                    double lam = A.number[b * F + f];
-                    s_lam[c] = lam;
-                    s_q1[c] = -expm1(-lam);
-                    s_el[c] = exp(-lam);
-                }
                 s_key[c] = sorted_mode ? (uint64_t)A.rank[b] : h_priority(h);
                 for (int k = 0; k < A.npar; k++) s_par[k][c] = A.par[((int64_t)k * A.B + b) * F + f];
             }
@@ -127,24 +114,7 @@ __global__ void __launch_bounds__(512) k_realize_chunk(RealizeArgs A) {
             if (!active) continue;
             for (int c = 0; c < ns; c++) {
                 const int64_t b = s0 + c;
-                double n;
-                if (INJECTED) {
-                    n = A.counts[(b * F + f) * R + r];
-                } else {
-                    double lam = s_lam[c];
-                    if (!(lam > 0.0)) continue;
-                    Stream st;
-                    st.init((uint32_t)b, (uint32_t)f, (uint32_t)r, 0u, A.seed);
-                    if (lam > A.thresh) {
-                        n = normal_count(st, lam);
-                    } else if (lam < 10.0) {
-                        double u = st.next();
-                        if (!(u < s_q1[c])) continue;   // n = 0: the common case
-                        n = poisson_inversion(u, lam, s_q1[c], s_el[c]);
-                    } else {
-                        n = poisson_ptrs(st, lam);
-                    }
-                }
+                const double n = A.counts[(b * F + f) * R + r];
                 const double h = s_h[c];
                 if (sorted_mode) {
                     if (n < 1) continue;
@@ -283,10 +253,10 @@ static size_t realize_ws(int64_t B, int F, int R, int L, int npar) {
     return s + 256;
 }
 
-static int realize_impl(bool injected, const double* number, const double* counts, const double* h2fdf,
-                        const int64_t* order, int64_t B, int F, int R, int L, int mode, int flags, double thresh,
-                        uint64_t seed, const double* par, int npar, double* hc2ss, int64_t* ssidx, double* hc2bg,
-                        double* bgpar, void* ws, size_t ws_bytes, hipStream_t st) {
+static int realize_counts_impl(const double* counts, const double* h2fdf, const int64_t* order, int64_t B, int F,
+                               int R, int L, int mode, int flags, const double* par, int npar, double* hc2ss,
+                               int64_t* ssidx, double* hc2bg, double* bgpar, void* ws, size_t ws_bytes,
+                               hipStream_t st) {
     if (B < 1 || F < 1 || R < 1 || L < 0 || !h2fdf || !hc2bg) return set_error(HD_ERR_ARG, "realize: bad sizes/pointers");
     if (B > 0x7fffffffll) return set_error(HD_ERR_SIZE, "realize: B=%lld exceeds 2^31-1", (long long)B);
     if (F > 65535) return set_error(HD_ERR_SIZE, "realize: F=%d > 65535", F);
@@ -298,13 +268,11 @@ static int realize_impl(bool injected, const double* number, const double* count
     if (npar < 0 || npar > NPAR_MAX || (npar > 0 && !par)) return set_error(HD_ERR_ARG, "realize: npar=%d invalid", npar);
     if (mode == HD_SEL_SORTED && !order) return set_error(HD_ERR_ARG, "realize: sorted mode needs an order");
     if (mode != HD_SEL_NONE && (!hc2ss || !ssidx)) return set_error(HD_ERR_ARG, "realize: hc2ss/ssidx required");
-    if (injected && !counts) return set_error(HD_ERR_ARG, "realize: counts required");
-    if (!injected && !number) return set_error(HD_ERR_ARG, "realize: number required");
+    if (!counts) return set_error(HD_ERR_ARG, "realize: counts required");
     size_t need = realize_ws(B, F, R, L, npar);
     if (!ws || ws_bytes < need) return set_error(HD_ERR_SIZE, "realize: workspace too small (%zu < %zu)", ws_bytes, need);
 
     RealizeArgs A;
-    A.number = number;
     A.counts = counts;
     A.h2fdf = h2fdf;
     A.par = par;
@@ -315,8 +283,6 @@ static int realize_impl(bool injected, const double* number, const double* count
     A.mode = mode;
     A.flags = flags;
     A.npar = npar;
-    A.thresh = thresh;
-    A.seed = seed;
     A.nchunks = (int)cdiv(B, RZ_CHUNK);
     size_t pr = (size_t)F * A.nchunks * R;
     char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
@@ -334,8 +300,7 @@ static int realize_impl(bool injected, const double* number, const double* count
     }
     int threads = (int)min((int64_t)512, cdiv(R, 64) * 64);
     dim3 grid(A.nchunks, F);
-    if (injected) hipLaunchKernelGGL(k_realize_chunk<true>, grid, dim3(threads), 0, st, A);
-    else return set_error(HD_ERR_ARG, "realize: the drawn-count path is hd_realize");
+    hipLaunchKernelGGL(k_realize_chunk, grid, dim3(threads), 0, st, A);
     int rc = check_launch("k_realize_chunk");
     if (rc) return rc;
     hipLaunchKernelGGL(k_realize_merge, dim3((unsigned)cdiv((int64_t)F * R, 256)), dim3(256), 0, st, A, hc2ss, ssidx,
@@ -349,7 +314,6 @@ int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int 
                          const int32_t* p_bin, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
                          hipStream_t st) {
     RealizeArgs A;
-    A.number = nullptr;
     A.counts = nullptr;
     A.h2fdf = h2fdf;
     A.rank = nullptr;
@@ -362,8 +326,6 @@ int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int 
     A.flags = 0;
     A.npar = npar;
     A.nchunks = nchunks;
-    A.thresh = 0.0;
-    A.seed = 0;
     A.p_total = const_cast<double*>(p_total);
     A.p_ptot = const_cast<double*>(p_ptot);
     A.p_key = const_cast<uint64_t*>(p_key);
@@ -390,8 +352,8 @@ extern "C" int hd_realize_counts(const double* counts, const double* h2fdf, cons
                                  int L, int mode, int flags, const double* par, int npar, double* hc2ss,
                                  int64_t* ssidx, double* hc2bg, double* bgpar, void* workspace, size_t ws_bytes,
                                  void* stream) {
-    return realize_impl(true, nullptr, counts, h2fdf, order, B, F, R, L, mode, flags, 0.0, 0, par, npar, hc2ss, ssidx,
-                        hc2bg, bgpar, workspace, ws_bytes, (hipStream_t)stream);
+    return realize_counts_impl(counts, h2fdf, order, B, F, R, L, mode, flags, par, npar, hc2ss, ssidx, hc2bg, bgpar,
+                               workspace, ws_bytes, (hipStream_t)stream);
 }
 
 extern "C" void hd_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {

@@ -424,7 +424,8 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
     sams_ = {}
 
     def prefetch(ii):
-        # the (host-only) SAM of row ii, and its grid's scatter geometry built by a worker thread
+        # the (host-only) SAM of row ii, and its grid's scatter geometry built by a worker process
+        nonlocal pool
         if ii >= S or ii in sams_:
             return
         shape = pspace.sam_shape if shapes is None else tuple(int(v) for v in shapes[ii])
@@ -438,7 +439,10 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
         key = _scatter.plan_key(sam.mtot, sam.mrat)
         if key in pending or _scatter.has_plan(sam.mtot, sam.mrat):
             return
-        pending[key] = pool.submit(_scatter.build_geometry, sam.mtot, sam.mrat, 4, 1)
+        try:
+            pending[key] = pool.submit(_scatter.build_geometry, sam.mtot, sam.mrat, 4, 1)
+        except Exception:           # a broken pool: geometry is then built in line when needed
+            pool = None
 
     streams_ = [torch.cuda.Stream(device=dev) for _ in range(max(1, int(streams)))]
     block = torch.empty((min(chunk, S), F, R), dtype=torch.float64, pin_memory=True)
@@ -458,8 +462,11 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
                 sam, settings = sams_.pop(ii)
                 fut = pending.pop(_scatter.plan_key(sam.mtot, sam.mrat), None)
                 if fut is not None:
-                    _scatter.put_plan(_scatter.ScatterPlan(sam.mtot, sam.mrat, geom=fut.result()), sam.mtot,
-                                      sam.mrat)
+                    try:
+                        geom = fut.result()
+                    except Exception:       # a broken worker pool (e.g. no importable __main__): in line
+                        geom, pool = None, None
+                    _scatter.put_plan(_scatter.ScatterPlan(sam.mtot, sam.mrat, geom=geom), sam.mtot, sam.mrat)
                 with torch.cuda.stream(streams_[ii % len(streams_)]):
                     hard = pspace._init_hard(sam, settings)
                     number, h2fdf, _ = sam._number_h2fdf_device(fobs_edges, hard)
