@@ -210,10 +210,12 @@ def eccen_harm_chunk(B, F, nharms, nreals):
 
 
 def gwb_eccen_discrete(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d, nharms, nreals, seed=0,
-                       harm_chunk=None, normal_threshold=NORMAL_THRESHOLD):
+                       harm_chunk=None, normal_threshold=float("inf")):
     """(F, nharms, R) characteristic strain squared with a Poisson count per (node, frequency,
-    harmonic, realization) (device).  Harmonics run in chunks; chunk c draws with the Philox seed
-    seed + (c+1) * 0x9E3779B97F4A7C15, so a result depends on (seed, harm_chunk)."""
+    harmonic, realization) (device).  Every count is Poisson, as the reference's random_poisson
+    call (cyutils.pyx:834) -- no Normal approximation unless ``normal_threshold`` is given.
+    Harmonics run in chunks; chunk c draws with the Philox seed seed + (c+1) * 0x9E3779B97F4A7C15,
+    so a result depends on (seed, harm_chunk)."""
     M, Q, Z, F, E = _eccen_args(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, sepa_d, eccen_d)
     H, R = int(nharms), int(nreals)
     hc = int(harm_chunk) if harm_chunk else eccen_harm_chunk(M * Q * Z, F, H, R)
