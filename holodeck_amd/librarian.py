@@ -494,6 +494,45 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
     return out if reduce is None else np.concatenate(out, axis=0)
 
 
+def eccen_library(pspace, eccen_inits, samples=None, shape=(61, 51, 61), pta_dur=DEF_PTA_DUR, nfreqs=DEF_NUM_FBINS,
+                  nharms=100, sepa_init=1.0 * PC, nsteps=123, cosmo=None, log_every=0):
+    """Eccentric-SAM spectra of parameter samples x initial eccentricities (as the reference's
+    eccentric library script, scripts/gen_lib_sams_eccen.py:744-790, with the deterministic
+    harmonic sum of cyutils.pyx:361-597 in place of its Poisson variant).
+
+    Per sample: the static binary density (with M-Mbulge scatter) on the device once; per initial
+    eccentricity: the GW-driven e(a) from ``sepa_init`` over ``nsteps`` (sams/sam.py:1235-1288)
+    and the (F, nharms) harmonic sum at the frequencies i/T.  Returns a dictionary with ``hc``
+    (S, E0, F) = sqrt of the harmonic-summed strain squared, ``sample_params`` (S, D),
+    ``eccen_init`` (E0,), ``fobs`` (F,) and ``samples_per_s``.
+    """
+    import torch
+    from . import engine, gravwaves
+    from .sams.sam import evolve_eccen_uniform_single
+    samples = np.arange(pspace.nsamples) if samples is None else np.asarray(samples)
+    e0s = np.asarray(eccen_inits, dtype=float)
+    fobs = utils.pta_freqs(dur=pta_dur * YR, num=int(nfreqs))[0]
+    if cosmo is None:
+        from .cosmology import cosmo as _cos
+        cosmo = _cos
+    hc = np.zeros((samples.size, e0s.size, fobs.size))
+    t0 = time.perf_counter()
+    for ii, snum in enumerate(samples):
+        sam, _ = pspace.model_for_sample_number(int(snum), sam_shape=tuple(shape), cosmo=cosmo)
+        outs = []
+        for e0 in e0s:
+            sepa, ecc = evolve_eccen_uniform_single(sam, float(e0), sepa_init, int(nsteps))
+            g = engine.gwb_eccen(*gravwaves._eccen_inputs(fobs, sam, sepa, ecc), int(nharms))
+            outs.append(torch.sqrt(g.sum(dim=1)))
+        hc[ii] = torch.stack(outs).cpu().numpy()
+        if log_every and (ii + 1) % log_every == 0:
+            dt = time.perf_counter() - t0
+            print(f"eccen_library: {ii + 1}/{samples.size} samples, {dt:.1f} s", flush=True)
+    return {'hc': hc, 'sample_params': pspace.param_samples[samples].copy(), 'eccen_init': e0s, 'fobs': fobs,
+            'param_names': np.array(pspace.param_names).astype('S'),
+            'samples_per_s': samples.size / (time.perf_counter() - t0)}
+
+
 def load_library(path):
     """Read a library written by :func:`run_library` (no pickled objects)."""
     with np.load(path, allow_pickle=False) as fh:

@@ -4,12 +4,16 @@
     python -m holodeck_amd.surrogate_cli train --library lib0.npz lib1.npz ... --out weights.npz
     python -m holodeck_amd.surrogate_cli evaluate --weights weights.npz --ninfer 100000
 
+    python -m holodeck_amd.surrogate_cli generate-eccen --nsamples 256 --ne0 20 --out ecc.npz
+
 ``generate`` runs Latin-hypercube samples of the classic uniform phenomenological space
 (librarian/param_spaces_classic.py:100-117) on one grid through the pipelined GPU SAM and
 stores parameters and log10 percentile targets (median, 5th, 95th h_c per frequency).  A
 library can be made in parts (``--part i --parts k`` takes every k-th sample from i).
-``train`` holds out a seeded random subset, fits the MLP with the HIP kernels and reports the
-held-out error in dex.  ``evaluate`` times inference on uniform random parameter vectors,
+``generate-eccen`` runs samples x initial eccentricities through the eccentric SAM (61x51x61,
+40 frequencies, 100 harmonics, deterministic harmonic sums) for a 7-input surrogate of log10 h_c.
+``train`` holds out a seeded random subset (whole samples with ``--group``), fits the MLP with
+the HIP kernels and reports the held-out error in dex.  ``evaluate`` times inference on uniform random parameter vectors,
 with the plain GPU SAM judging a fraction of them.
 """
 import argparse
@@ -39,6 +43,23 @@ def cmd_generate(a):
                       "grid": list(a.shape), "nfreqs": a.nfreqs, "nreals": a.nreals}), flush=True)
 
 
+def cmd_generate_eccen(a):
+    """Eccentric library for a 7-input surrogate: samples x initial eccentricities, targets log10
+    h_c at each frequency (deterministic harmonic sums)."""
+    ps = _space(a.nsamples, a.seed, a.shape)
+    e0 = np.linspace(0.0, 0.99, a.ne0)
+    lib = librarian.eccen_library(ps, e0, shape=a.shape, nfreqs=a.nfreqs, nharms=a.nharms, log_every=a.log_every)
+    S, E, F = lib['hc'].shape
+    X = np.concatenate([np.repeat(lib['sample_params'], E, axis=0), np.tile(e0, S)[:, None]], axis=1)
+    Y = np.log10(np.maximum(lib['hc'].reshape(S * E, F), surrogate.FLOOR))
+    extrema = np.vstack([ps.extrema, [[0.0, 0.99]]])
+    np.savez(a.out, sample_params=X, targets=Y.astype(np.float32),
+             param_names=np.array(list(ps.param_names) + ['eccen_init']).astype('S'), fobs_cents=lib['fobs'],
+             nreals=np.array(0), shape=np.array(a.shape), extrema=extrema, nharms=np.array(a.nharms))
+    print(json.dumps({"generated": int(S * E), "samples": int(S), "eccen_inits": int(E),
+                      "samples_per_s": lib['samples_per_s'], "spectra_per_s": lib['samples_per_s'] * E}), flush=True)
+
+
 def _load(paths):
     X, Y, meta = [], [], None
     for p in paths:
@@ -53,8 +74,13 @@ def _load(paths):
 def cmd_train(a):
     X, Y, meta = _load(a.library)
     rng = np.random.default_rng(a.seed)
-    perm = rng.permutation(X.shape[0])
-    hold, train = perm[:a.holdout], perm[a.holdout:]
+    # hold out whole groups of consecutive rows (a parameter sample's rows when a library has
+    # several per sample, e.g. one per initial eccentricity)
+    g = max(1, int(a.group))
+    groups = rng.permutation(X.shape[0] // g)
+    nh = max(1, a.holdout // g)
+    rows = np.arange(X.shape[0]).reshape(-1, g)
+    hold, train = rows[groups[:nh]].ravel(), rows[groups[nh:]].ravel()
     net = surrogate.MLPSurrogate(n_in=X.shape[1], hidden=[a.width] * a.depth, n_out=Y.shape[1], seed=a.seed)
     ext = meta['extrema']
     t0 = time.perf_counter()
@@ -62,12 +88,13 @@ def cmd_train(a):
                    x_lo=ext[:, 0], x_hi=ext[:, 1], verbose=a.log_every)
     t_fit = time.perf_counter() - t0
     err = np.abs(net.predict(X[hold]) - Y[hold])
-    F = Y.shape[1] // 3
     rep = {"train": int(train.size), "holdout": int(hold.size), "epochs": a.epochs, "fit_s": t_fit,
            "final_loss": float(hist[-1]), "median_dex_error": float(np.median(err)),
-           "p95_dex_error": float(np.percentile(err, 95)),
-           "median_dex_error_by_output": {name: float(np.median(err[:, k * F:(k + 1) * F]))
-                                          for k, name in enumerate(("p50", "p05", "p95"))}}
+           "p95_dex_error": float(np.percentile(err, 95))}
+    if int(meta['nreals']) > 0:                 # percentile targets: [median | p05 | p95] per frequency
+        F = Y.shape[1] // 3
+        rep["median_dex_error_by_output"] = {name: float(np.median(err[:, k * F:(k + 1) * F]))
+                                             for k, name in enumerate(("p50", "p05", "p95"))}
     net.save(a.out, param_names=meta['param_names'], fobs_cents=meta['fobs_cents'], nreals=meta['nreals'],
              shape=meta['shape'], holdout_median_dex_error=rep["median_dex_error"],
              holdout_p95_dex_error=rep["p95_dex_error"], ntrain=train.size)
@@ -103,9 +130,19 @@ def main(argv=None):
     g.add_argument("--streams", type=int, default=3)
     g.add_argument("--log-every", type=int, default=256)
     g.add_argument("--out", required=True)
+    ge = sub.add_parser("generate-eccen")
+    ge.add_argument("--nsamples", type=int, default=256)
+    ge.add_argument("--seed", type=int, default=5150)
+    ge.add_argument("--ne0", type=int, default=20)
+    ge.add_argument("--shape", type=int, nargs=3, default=(61, 51, 61))
+    ge.add_argument("--nfreqs", type=int, default=40)
+    ge.add_argument("--nharms", type=int, default=100)
+    ge.add_argument("--log-every", type=int, default=32)
+    ge.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--library", nargs="+", required=True)
     t.add_argument("--holdout", type=int, default=1000)
+    t.add_argument("--group", type=int, default=1, help="rows per parameter sample (held out together)")
     t.add_argument("--width", type=int, default=256)
     t.add_argument("--depth", type=int, default=4)
     t.add_argument("--epochs", type=int, default=2000)
@@ -121,7 +158,8 @@ def main(argv=None):
     e.add_argument("--judge-frac", type=float, default=0.01)
     e.add_argument("--seed", type=int, default=7)
     a = ap.parse_args(argv)
-    {"generate": cmd_generate, "train": cmd_train, "evaluate": cmd_evaluate}[a.cmd](a)
+    {"generate": cmd_generate, "generate-eccen": cmd_generate_eccen, "train": cmd_train,
+     "evaluate": cmd_evaluate}[a.cmd](a)
 
 
 if __name__ == "__main__":

@@ -91,3 +91,21 @@ def test_pipelined_gwb_library_matches_run_library(gpu):
     assert lib["gwb"].shape == (7, 5, 16) and lib["samples_per_s"] > 0
     assert np.array_equal(lib["gwb"], ref["gwb"])
     assert np.array_equal(tg["med"], np.median(ref["gwb"], axis=-1))
+
+
+@pytest.mark.gpu
+def test_eccentric_library_matches_the_per_model_call(gpu):
+    """eccen_library's spectra are the harmonic-summed eccentric GWB of each (sample, e0) model."""
+    print("[eccen library]", flush=True)
+    from holodeck_amd import gravwaves
+    from holodeck_amd.sams.sam import evolve_eccen_uniform_single
+    from holodeck_amd.constants import PC, YR
+    ps = L.PS_Classic_Phenom_Uniform(nsamples=2, seed=4)
+    e0s = [0.0, 0.5, 0.9]
+    lib = L.eccen_library(ps, e0s, shape=(13, 9, 11), nfreqs=5, nharms=10)
+    assert lib['hc'].shape == (2, 3, 5) and np.all(lib['hc'] > 0)
+    sam, _ = ps.model_for_sample_number(1, sam_shape=(13, 9, 11))
+    sepa, ecc = evolve_eccen_uniform_single(sam, 0.5, 1.0 * PC, 123)
+    g = gravwaves.sam_calc_gwb_single_eccen(lib['fobs'], sam, sepa, ecc, nharms=10)
+    assert np.allclose(lib['hc'][1, 1], np.sqrt(g.sum(axis=1)), rtol=1e-12)
+    assert np.allclose(lib['fobs'], np.arange(1, 6) / (L.DEF_PTA_DUR * YR))
