@@ -30,7 +30,7 @@ def main(argv=None):
     ap.add_argument("--nreals", type=int, default=100)
     ap.add_argument("--nloudest", type=int, default=5)
     ap.add_argument("--pta-dur", type=float, default=librarian.DEF_PTA_DUR, help="[yr]")
-    ap.add_argument("--streams", type=int, default=3)
+    ap.add_argument("--streams", default="auto", help="HIP streams, or 'auto' (timed on the first samples)")
     ap.add_argument("--full", action="store_true", help="run_model per sample (gwb, hc_ss, hc_bg)")
     ap.add_argument("--log-every", type=int, default=256)
     ap.add_argument("--out", default=None, help="write the library (.npz)")
@@ -42,10 +42,13 @@ def main(argv=None):
         lib = librarian.run_library(ps, shapes=shapes, pta_dur=a.pta_dur, nfreqs=a.nfreqs, nreals=a.nreals,
                                     nloudest=a.nloudest, seed=a.seed, log_every=a.log_every)
         rate = a.nsamples / (time.perf_counter() - t0)
+        used = 1
     else:
+        streams = a.streams if a.streams == "auto" else int(a.streams)
         lib = librarian.gwb_library(ps, shapes=shapes, pta_dur=a.pta_dur, nfreqs=a.nfreqs, nreals=a.nreals,
-                                    seed=a.seed, streams=a.streams, log_every=a.log_every)
+                                    seed=a.seed, streams=streams, log_every=a.log_every)
         rate = lib.pop('samples_per_s')
+        used = int(lib.pop('streams_used'))
     if shapes is not None:
         lib['sam_shapes'] = shapes
     cells = (np.prod(shapes - 1, axis=1).sum() if shapes is not None
@@ -55,7 +58,7 @@ def main(argv=None):
     if a.out:
         np.savez(a.out, **lib)
     print(json.dumps({"samples": a.nsamples, "samples_per_s": rate, "uneven": bool(a.uneven),
-                      "nfreqs": a.nfreqs, "nreals": a.nreals, "full": bool(a.full), "finite": ok,
+                      "nfreqs": a.nfreqs, "nreals": a.nreals, "full": bool(a.full), "finite": ok, "streams": used,
                       "cell_realizations_per_s": float(cells) * a.nfreqs * a.nreals * rate / a.nsamples,
                       "median_hc_f0": float(np.median(gwb[:, 0, :]))}), flush=True)
     return 0 if ok else 1

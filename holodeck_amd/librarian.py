@@ -370,19 +370,24 @@ def gwb_library(pspace, shapes=None, pta_dur=DEF_PTA_DUR, nfreqs=DEF_NUM_FBINS, 
     fobs_cents, fobs_edges = utils.pta_freqs(dur=pta_dur * YR, num=int(nfreqs))
     params = pspace.param_samples[samples]
     t0 = time.perf_counter()
+    info = {}
     out = gwb_for_params(pspace, params, shapes=shapes, fobs_edges=fobs_edges, nreals=nreals,
                          seeds=seed + 2 * samples.astype(np.int64) + 1, cosmo=cosmo, streams=streams, chunk=chunk,
-                         reduce=reduce, plan_workers=plan_workers, lookahead=lookahead, log_every=log_every)
+                         reduce=reduce, plan_workers=plan_workers, lookahead=lookahead, log_every=log_every, info=info)
     lib = {'fobs_cents': fobs_cents, 'fobs_edges': fobs_edges, 'sample_params': params.copy(),
            ('gwb' if reduce is None else reduce_key): out,
-           'samples_per_s': len(samples) / (time.perf_counter() - t0),
+           'samples_per_s': len(samples) / (time.perf_counter() - t0), 'streams_used': np.array(info['streams']),
            'param_names': np.array(pspace.param_names).astype('S'),
            'parameter_space_class_name': np.array(pspace.name)}
     return lib
 
 
+AUTOTUNE_STREAMS = (2, 3, 4)     # stream counts tried by streams='auto'
+AUTOTUNE_PROBE = 16              # rows timed per candidate
+
+
 def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_REALS, seeds=0, cosmo=None,
-                   streams=3, chunk=256, reduce=None, plan_workers=8, lookahead=16, log_every=0):
+                   streams=3, chunk=256, reduce=None, plan_workers=8, lookahead=16, log_every=0, info=None):
     """Poisson GWB h_c (n, F, R) for parameter rows ``params`` (n, D) of ``pspace`` (unvaried
     parameters from its ``DEFAULTS``), or ``reduce`` of it per block of samples, concatenated.
 
@@ -395,6 +400,11 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
     before is built by ``plan_workers`` host processes up to ``lookahead`` rows ahead (0: in line,
     when a row needs it).  ``seeds``: one Philox seed per row, or a base seed ``s`` giving row
     ``i`` the seed ``s + i``.
+
+    ``streams='auto'``: the first rows are processed in blocks of AUTOTUNE_PROBE rows, one block
+    per candidate in AUTOTUNE_STREAMS, each timed to completion; the fastest count runs the rest
+    (rows are never recomputed: a row's result does not depend on its stream).  ``info``, if a
+    dict, receives the chosen count and the probe rates.
     """
     import torch
     from . import engine, _lib, scatter as _scatter
@@ -444,18 +454,27 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
         except Exception:           # a broken pool: geometry is then built in line when needed
             pool = None
 
-    streams_ = [torch.cuda.Stream(device=dev) for _ in range(max(1, int(streams)))]
-    block = torch.empty((min(chunk, S), F, R), dtype=torch.float64, pin_memory=True)
+    auto = isinstance(streams, str) and streams == 'auto'
+    cands = list(AUTOTUNE_STREAMS) if auto and S >= 4 * AUTOTUNE_PROBE * len(AUTOTUNE_STREAMS) else []
+    nstream = max(cands) if cands else (3 if auto else max(1, int(streams)))
+    streams_ = [torch.cuda.Stream(device=dev) for _ in range(max([nstream] + cands))]
+    block = torch.empty((min(max(chunk, AUTOTUNE_PROBE), S), F, R), dtype=torch.float64, pin_memory=True)
     out = np.zeros((S, F, R)) if reduce is None else []
     inflight = collections.deque()      # one event per launched row: the host stays <= depth rows ahead
-    depth = 2 * len(streams_)
+    probe_rates = {}
     t0 = time.perf_counter()
     try:
         for ii in range(min(lookahead, S)):
             prefetch(ii)
         start = 0
         while start < S:
-            n = min(chunk, S - start)
+            if cands:                           # autotuning block: the next candidate stream count
+                nstream = cands[len(probe_rates)]
+                n = AUTOTUNE_PROBE
+                tb = time.perf_counter()
+            else:
+                n = min(chunk, S - start)
+            depth = 2 * nstream
             for j in range(n):
                 ii = start + j
                 prefetch(ii + lookahead)
@@ -467,7 +486,7 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
                     except Exception:       # a broken worker pool (e.g. no importable __main__): in line
                         geom, pool = None, None
                     _scatter.put_plan(_scatter.ScatterPlan(sam.mtot, sam.mrat, geom=geom), sam.mtot, sam.mrat)
-                with torch.cuda.stream(streams_[ii % len(streams_)]):
+                with torch.cuda.stream(streams_[ii % nstream]):
                     hard = pspace._init_hard(sam, settings)
                     number, h2fdf, _ = sam._number_h2fdf_device(fobs_edges, hard)
                     _, _, hc2, _ = engine.realize(number, h2fdf, R, 0, _lib.HD_SEL_NONE, seed=int(seeds[ii]))
@@ -479,6 +498,11 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
                     inflight.popleft().synchronize()
             torch.cuda.synchronize()
             inflight.clear()
+            if cands:
+                probe_rates[nstream] = n / (time.perf_counter() - tb)
+                if len(probe_rates) == len(cands):
+                    nstream = max(probe_rates, key=probe_rates.get)
+                    cands = []
             hc = np.sqrt(block[:n].numpy())
             if reduce is None:
                 out[start:start + n] = hc
@@ -491,6 +515,8 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
     finally:
         for fut in pending.values():
             fut.cancel()
+    if isinstance(info, dict):
+        info.update(streams=nstream, probe_rates=probe_rates)
     return out if reduce is None else np.concatenate(out, axis=0)
 
 

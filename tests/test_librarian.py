@@ -109,3 +109,17 @@ def test_eccentric_library_matches_the_per_model_call(gpu):
     g = gravwaves.sam_calc_gwb_single_eccen(lib['fobs'], sam, sepa, ecc, nharms=10)
     assert np.allclose(lib['hc'][1, 1], np.sqrt(g.sum(axis=1)), rtol=1e-12)
     assert np.allclose(lib['fobs'], np.arange(1, 6) / (L.DEF_PTA_DUR * YR))
+
+
+@pytest.mark.gpu
+def test_stream_autotuning_keeps_results(gpu):
+    """streams='auto' times each candidate on real rows and keeps the fastest; the results equal a
+    fixed-stream run bit for bit."""
+    print("[autotune]", flush=True)
+    ps = L.PS_Classic_Phenom_Uniform(nsamples=200, sam_shape=(13, 11, 9), seed=2)
+    info = {}
+    a = L.gwb_for_params(ps, ps.param_samples, fobs_edges=np.arange(1, 6) / 5e8, nreals=8, seeds=3, streams='auto',
+                         chunk=64, info=info)
+    b = L.gwb_for_params(ps, ps.param_samples, fobs_edges=np.arange(1, 6) / 5e8, nreals=8, seeds=3, streams=1)
+    assert info["streams"] in L.AUTOTUNE_STREAMS and len(info["probe_rates"]) == len(L.AUTOTUNE_STREAMS)
+    assert np.array_equal(a, b)
