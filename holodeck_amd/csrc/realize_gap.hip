@@ -482,14 +482,19 @@ __device__ __forceinline__ double gz_u52(uint32_t hi, uint32_t lo) {
 // Inversion: the smallest k in the window with CDF(k) > u (branch-free binary
 // search); a uniform above the table's last value (CDF rounding near 1)
 // returns the window's end.
+// (The candidate is carried as an address, so a step costs one add, one compare and one
+// select.)
 __device__ __forceinline__ double gz_table_draw(const double* tab, int klo, int n, double u) {
-    int base = 0, len = n;
-    while (len > 1) {   // invariant: the answer is in [base, base + len]
+    const double* p = tab;   // invariant: the answer is in [p - tab, p - tab + len]
+    int len = n;
+    while (len > 1) {
         const int half = len >> 1;
-        base = (tab[base + half] <= u) ? base + half : base;
+        const double* q = p + half;
+        p = (*q <= u) ? q : p;
         len -= half;
     }
-    return (double)(klo + min(base + (tab[base] <= u ? 1 : 0), n - 1));
+    const int base = (int)(p - tab);
+    return (double)(klo + min(base + (*p <= u ? 1 : 0), n - 1));
 }
 
 // Counts without a table: Normal above the threshold, inversion below 10,
@@ -595,19 +600,24 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                     // even record, (z, w) for the odd one; counter (tile, f, r, tag 5 | pair).
                     int blk_pair = -1;
                     U4 blk = {0u, 0u, 0u, 0u};
-                    for (int i = c; i < c1; i++) {
+                    // dense records of the batch (one control path, so the running list needs no
+                    // copies between branches), then the tile's sparse pseudo-record if it is here
+                    const int cd = min(c1, nd);
+                    for (int i = c; i < cd; i++) {
                         const BRec rec = brec[i - c];
-                        if (i < nd) {
-                            if (rec.n == 0) continue;   // no table: drawn by k_gz_big
-                            if ((i >> 1) != blk_pair) {
-                                blk_pair = i >> 1;
-                                blk = philox4x32_10(U4{(uint32_t)tile, (uint32_t)f, (uint32_t)r,
-                                                       (5u << 16) | (uint32_t)blk_pair}, k0, k1);
-                            }
-                            const double u = (i & 1) ? gz_u52(blk.z, blk.w) : gz_u52(blk.x, blk.y);
-                            const double n = gz_table_draw(tab + rec.off, rec.klo, rec.n, u);
-                            acc.add(n, rec.h, rec.word, A.d_par + reg + i, pstride, A.npar, L);
-                        } else {
+                        if (rec.n == 0) continue;   // no table: drawn by k_gz_big
+                        if ((i >> 1) != blk_pair) {
+                            blk_pair = i >> 1;
+                            blk = philox4x32_10(U4{(uint32_t)tile, (uint32_t)f, (uint32_t)r,
+                                                   (5u << 16) | (uint32_t)blk_pair}, k0, k1);
+                        }
+                        const double u = (i & 1) ? gz_u52(blk.z, blk.w) : gz_u52(blk.x, blk.y);
+                        const double n = gz_table_draw(tab + rec.off, rec.klo, rec.n, u);
+                        acc.add(n, rec.h, rec.word, A.d_par + reg + i, pstride, A.npar, L);
+                    }
+                    for (int i = max(c, nd); i < c1; i++) {
+                        const BRec rec = brec[i - c];
+                        {
                             // sparse bins: Poisson process on the tile's cumulative-lambda axis; the
                             // point count from stream tag 4 (its window always fits: lambda < 512),
                             // the positions from stream tag 2
