@@ -37,19 +37,16 @@ def test_full_size_gwb_properties(gpu):
     assert np.all(np.diff(ss[0], axis=-1) <= 0)
     # loud + background = the total of the same draws (counts are integers below 1e10)
     assert np.allclose(bg + ss.sum(axis=-1), tot, rtol=1e-12, atol=0)
-    # the mean over realizations is the expectation sum(lambda h), with the exact variance
-    # sum(lambda h^2) / R -- at the frequencies where no single bin carries more than 1% of that
-    # variance, so that the mean of 500 realizations is close to normal
+    # the mean over realizations is the expectation sum(lambda h), to within 5 standard errors of
+    # the exact variance sum(lambda h^2) / R.  That variance is carried by rare loud bins
+    # (lambda < 0.01), so the bound is loose at high frequencies; at frequency 0 it still catches a
+    # 5% error in the mean.
     lam_h = number * h2fdf
     expect = D.to_host(lam_h.sum(dim=0))
-    var_b = lam_h * h2fdf
-    var = D.to_host(var_b.sum(dim=0))
-    share = D.to_host(var_b.max(dim=0).values) / var
-    use = share < 0.01
-    assert use.sum() >= 3, share
+    var = D.to_host((lam_h * h2fdf).sum(dim=0))
     z = (tot.mean(axis=1) - expect) / np.sqrt(var / R)
-    print(f"z over {int(use.sum())} frequencies: max |z| = {np.abs(z[use]).max():.2f}", flush=True)
-    assert np.all(np.abs(z[use]) < 5), z[use]
+    assert np.all(np.abs(z) < 5), z
+    assert abs(tot[0].mean() / expect[0] - 1.0) < 0.05
     # a seed repeats bit for bit
     ss2, idx2, bg2, _ = engine.realize(number, h2fdf, R, L, _lib.HD_SEL_SORTED, order=order, seed=21)
     assert np.array_equal(D.to_host(ss2), ss) and np.array_equal(D.to_host(bg2), bg)
