@@ -6,7 +6,6 @@ unseeded) and ``counts`` (inject the per-(bin, frequency, realization) counts,
 shape number.shape + (R,), instead of drawing them).
 """
 import numpy as np
-import torch
 
 from . import _lib
 from . import device as D

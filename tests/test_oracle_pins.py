@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 
 import oracle
-from oracle.pipeline import OracleCosmo, MSOL, PC, GYR, YR, SPLC, MPC, NWTG
+from oracle.pipeline import OracleCosmo, MSOL, PC, GYR, YR, SPLC, MPC
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gw_formulas.json")
 
