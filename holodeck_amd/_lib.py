@@ -56,6 +56,8 @@ SIGNATURES = {
     "hd_mlp_colsum_bf16": (_I32, [_P, _I32, _I32, _P, _P]),
     "hd_mlp_mse": (_I32, [_P, _P, _I64, _P, _P, _P]),
     "hd_mlp_adam": (_I32, [_I64, _P, _P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float, _I32, _P, _P]),
+    "hd_gwb_ideal_workspace_size": (_SZ, []),
+    "hd_gwb_ideal": (_I32, [_P, _P, _P, _I32, _P, _I32, _P, _I32, _I32, _P, _I32, _D, _D, _D, _I32, _P, _P, _SZ, _P]),
     "hd_eccen_workspace_size": (_SZ, [_I32, _I32, _I32]),
     "hd_gwb_eccen": (_I32, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _P, _P, _SZ, _P]),
     "hd_eccen_number": (_I32, [_P, _P, _I32, _P, _I32, _P, _P, _I32, _P, _I32, _P, _P, _I32, _I32, _I32, _P, _P, _P,

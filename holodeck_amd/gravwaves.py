@@ -69,3 +69,39 @@ def sam_calc_gwb_single_eccen_discrete(gwfobs, sam, sepa_evo, eccen_evo, nharms=
     gwb = D.to_host(engine.gwb_eccen_discrete(*_eccen_inputs(gwfobs, sam, sepa_evo, eccen_evo), int(nharms), R,
                                               seed=_seed(seed), harm_chunk=harm_chunk))
     return gwb.squeeze() if squeeze else gwb
+
+
+def _gwb_ideal_device(fobs_gw, dens, log10_mtot_d, mtot_d, mrat_d, redz_d, sum=True):
+    """Device evaluation of the idealised GWB (gravwaves.py:619-663); dens (M,Q,Z) [Mpc^-3],
+    redz_d (Z) or (M,Q,Z)."""
+    from .constants import NWTG, SPLC, MPC
+    M, Q, Z = dens.shape
+    fo = D.to_dev(np.atleast_1d(np.asarray(fobs_gw, dtype=float)))
+    F = fo.numel()
+    rz3 = redz_d.dim() == 3
+    if sum:
+        ws = D.workspace(_lib.size("hd_gwb_ideal_workspace_size"))
+        out = D.empty(F)
+    else:
+        ws = None
+        out = D.empty((M - 1, Q - 1, Z - 1, F))
+    _lib.call("hd_gwb_ideal", D.ptr(dens), D.ptr(log10_mtot_d), D.ptr(mtot_d), M, D.ptr(mrat_d), Q,
+              D.ptr(redz_d.contiguous()), int(rz3), Z, D.ptr(fo), F, float(NWTG), float(SPLC), float(MPC), int(bool(sum)),
+              D.ptr(out), D.ptr(ws), 0 if ws is None else ws.numel(), D.stream_ptr())
+    res = D.to_host(out)
+    return res if np.ndim(fobs_gw) else res.reshape(res.shape[:-1])
+
+
+def gwb_ideal(fobs_gw, ndens, mtot, mrat, redz, dlog10, sum=True):
+    """Idealised GWB h_c (gravwaves.py:619-663): ``ndens`` (M,Q,Z) [cm^-3] at the grid nodes,
+    ``mtot`` (M,1,1) / ``mrat`` (1,Q,1) edges, ``redz`` (1,1,Z) edges or (M,Q,Z) node redshifts."""
+    from .constants import MPC
+    if not dlog10:
+        raise NotImplementedError("only the SAM's integration in log10 of the total mass is provided")
+    ndens = np.asarray(ndens, dtype=float)
+    mtot = np.ravel(mtot)
+    mrat = np.ravel(mrat)
+    redz = np.asarray(redz, dtype=float)
+    rz = redz if redz.shape == ndens.shape else np.ravel(redz)
+    return _gwb_ideal_device(fobs_gw, D.to_dev(ndens * MPC ** 3), D.to_dev(np.log10(mtot)), D.to_dev(mtot),
+                             D.to_dev(mrat), D.to_dev(rz), sum=sum)

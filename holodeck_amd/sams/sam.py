@@ -234,7 +234,20 @@ class Semi_Analytic_Model:
                 raise ValueError(f"After 0th order interpolation, {bad} bad values remain!")
 
     def gwb_ideal(self, fobs_gw, sum=True, redz_prime=None):
-        raise NotImplementedError("gwb_ideal is not part of the device path")
+        """Idealised, continuous GWB h_c at GW frequencies (sam.py:837-870; Phinney 2001 Eq. 5):
+        purely GW-driven, no finite-number effects, no coalescence cut.  With a GMT (or
+        ``redz_prime=True``) the post-merger redshifts z' are used.  Returns (F,) or, with
+        ``sum=False``, (M-1, Q-1, Z-1, F)."""
+        from .. import gravwaves
+        if redz_prime is None:
+            redz_prime = (self._gmt is not None)
+        elif redz_prime and (self._gmt is None):
+            raise AttributeError("No `GMT` instance stored, cannot use `redz_prime` values!")
+        dens, zp, _ = self._density_device()
+        self._check_scatter()
+        g = self._grids()
+        rz = zp if redz_prime else g['redz']
+        return gravwaves._gwb_ideal_device(fobs_gw, dens, g['log10_mtot'], g['mtot'], g['mrat'], rz, sum=sum)
 
 
 def evolve_eccen_uniform_single(sam, eccen_init, sepa_init, nsteps):

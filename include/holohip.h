@@ -148,6 +148,18 @@ int hd_bin_params_redz(const double *mt_cent, int M, const double *mr_cent, int 
                        const double *fc, int F, const double *redz_final, double om0, double h,
                        const double *dc_tab, const double *dc_der, int n_tab, double *out, void *stream);
 
+/* <- gravwaves.gwb_ideal (holodeck/gravwaves.py:619-663) for Semi_Analytic_Model.gwb_ideal
+ * (holodeck/sams/sam.py:837-870): the idealised GWB h_c(f) of a continuous, GW-driven
+ * population.  ndens (M,Q,Z) [Mpc^-3] at the grid nodes, log10 and linear total-mass edges [g],
+ * mass-ratio edges, redshifts: (Z) edges, or with redz_3d != 0 a per-node z' (M,Q,Z) (the
+ * reference's redz_prime); constants G, c [cgs] and Mpc [cm] as the caller's module defines
+ * them.  sum != 0: out (F) h_c; workspace of hd_gwb_ideal_workspace_size() bytes.
+ * sum == 0: out (M-1,Q-1,Z-1,F) per-cell h_c, no workspace. */
+size_t hd_gwb_ideal_workspace_size(void);
+int hd_gwb_ideal(const double *ndens, const double *log10_mtot, const double *mtot, int M, const double *mrat, int Q,
+                 const double *redz, int redz_3d, int Z, const double *fobs_gw, int F, double nwtg, double splc,
+                 double mpc, int sum, double *out, void *workspace, size_t ws_bytes, void *stream);
+
 /* ------------------------------------------------------------------------
  * Ordering  <- single_sources.ss_gws_redz's argsort(-h2fdf[..., 0])
  * (single_sources.py:89-93), made stable: descending by key, ties by
