@@ -30,6 +30,7 @@ SIGNATURES = {
     "hd_cosmo_eval": (_I32, [_D, _D, _P, _P, _I32, _P, _I64, _P, _P, _P, _P]),
     "hd_static_density": (_I32, [_P, _I32, _P, _I32, _P, _I32, _P, _P, _P, _I32, _P, _P, _P, _P]),
     "hd_zero_stalled": (_I32, [_P, _P, _I64, _P]),
+    "hd_mass_stellar": (_I32, [_P, _I32, _P, _I32, _I32, _P, _P, _P, _P, _P]),
     "hd_find_2pwl_norm": (_I32, [_D, _P, _P, _I64, _D, _D, _D, _D, _I32, _P, _P, _P]),
     "hd_hard_func_2pwl_gw": (_I32, [_P, _P, _P, _P, _I64, _D, _D, _D, _P, _P]),
     "hd_dbn_gw": (_I32, [_P, _I32, _P, _P, _I32, _P, _I32, _I32, _P, _P, _P, _I32, _P, _P, _P]),

@@ -130,6 +130,39 @@ extern "C" int hd_static_density(const double* mtot, int M, const double* mrat, 
     return check_launch("k_static_density");
 }
 
+namespace hd {
+// mass_stellar (sam.py:250-278): per (M, q) node, broadcast over the Z redshifts as the reference
+__global__ void __launch_bounds__(256) k_mass_stellar(const double* __restrict__ mtot, int M,
+                                                      const double* __restrict__ mrat, int Q, int Z, DensPar DP,
+                                                      double* __restrict__ mpri, double* __restrict__ mrt,
+                                                      double* __restrict__ mtt) {
+    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= (int64_t)M * Q * Z) return;
+    const int jj = (int)((cell / Z) % Q);
+    const int ii = (int)(cell / ((int64_t)Q * Z));
+    const double mt = mtot[ii], mr = mrat[jj];
+    const double m1 = mt / (1.0 + mr);
+    const double m2 = mt - m1;
+    const double sp = mstar_from_mbh(DP.p, m1);
+    const double ss = mstar_from_mbh(DP.p, m2);
+    mpri[cell] = sp;
+    mrt[cell] = ss / sp;
+    mtt[cell] = sp + ss;
+}
+}  // namespace hd
+
+extern "C" int hd_mass_stellar(const double* mtot, int M, const double* mrat, int Q, int Z, const double* par,
+                               double* mstar_pri, double* mstar_rat, double* mstar_tot, void* stream) {
+    if (M < 1 || Q < 1 || Z < 1 || !mtot || !mrat || !par || !mstar_pri || !mstar_rat || !mstar_tot)
+        return set_error(HD_ERR_ARG, "hd_mass_stellar: bad arguments");
+    DensPar DP;
+    for (int i = 0; i < HD_DENS_NPAR; i++) DP.p[i] = par[i];   // host array
+    const int64_t n = (int64_t)M * Q * Z;
+    hipLaunchKernelGGL(k_mass_stellar, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, mtot, M, mrat,
+                       Q, Z, DP, mstar_pri, mstar_rat, mstar_tot);
+    return check_launch("k_mass_stellar");
+}
+
 extern "C" int hd_zero_stalled(double* dens, const double* zprime, int64_t n, void* stream) {
     if (n < 0 || (n > 0 && (!dens || !zprime))) return set_error(HD_ERR_ARG, "hd_zero_stalled: bad arguments");
     if (n == 0) return HD_OK;

@@ -228,3 +228,10 @@ def gwb_eccen_discrete(ndens, mtot_log10_d, mrat_d, redz_d, dcom_d, gwfobs_d, se
         out[:, h0:h0 + n, :] = hc2.view(F, n, R)
         del lam, hv, hc2
     return out
+
+
+def mass_stellar(mtot_d, M, mrat_d, Q, Z, par, mpri, mrat_out, mtot_out):
+    """Stellar masses per (M, Q, Z) node into the three device arrays (sam.py:250-278)."""
+    par = np.ascontiguousarray(par, dtype=np.float64)
+    _lib.call("hd_mass_stellar", D.ptr(mtot_d), M, D.ptr(mrat_d), Q, Z, par.ctypes.data, D.ptr(mpri), D.ptr(mrat_out),
+              D.ptr(mtot_out), D.stream_ptr())

@@ -101,6 +101,19 @@ class Semi_Analytic_Model:
                              mr_cent=D.to_dev(utils.midpoints(self.mrat)))
         return self._dev
 
+    def mass_stellar(self):
+        """(mstar_pri, mstar_rat, mstar_tot, redz), each (M, Q, Z), from the M-Mbulge relation
+        without scatter (sam.py:250-278); computed on the device, returned as numpy."""
+        from .. import engine
+        g = self._grids()
+        par = density_params(self._gsmf, self._gpf, self._gmt, self._gmr, self._mmbulge, self._cosmo)
+        M, Q, Z = self.shape
+        out = [D.empty((M, Q, Z)) for _ in range(3)]
+        engine.mass_stellar(g['mtot'], M, g['mrat'], Q, Z, par, *out)
+        mp, mr, mt = (D.to_host(t) for t in out)
+        redz = np.broadcast_to(self.redz[np.newaxis, np.newaxis, :], (M, Q, Z))
+        return mp, mr, mt, redz
+
     # ---- density ----
     def _density_device(self):
         """Device (dens, zprime|None, gmt_time|None): static_binary_density (sam.py:280-398)."""

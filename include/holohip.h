@@ -71,6 +71,11 @@ int hd_static_density(const double *mtot, int M, const double *mrat, int Q, cons
                       const double *par, const double *dc_tab, const double *dc_der, int n_tab,
                       double *dens, double *zprime, double *gmt_time, void *stream);
 /* dens[zprime < 0] = 0  (sam.py:392-394) */
+/* <- Semi_Analytic_Model.mass_stellar (holodeck/sams/sam.py:250-278): stellar masses of the
+ * primary, their ratio and total per (M,Q,Z) node (redshift-independent, broadcast over Z) from
+ * the M-Mbulge parameters in `par` (the host array of hd_static_density). */
+int hd_mass_stellar(const double *mtot, int M, const double *mrat, int Q, int Z, const double *par, double *mstar_pri,
+                    double *mstar_rat, double *mstar_tot, void *stream);
 int hd_zero_stalled(double *dens, const double *zprime, int64_t n, void *stream);
 
 /* ------------------------------------------------------------------------

@@ -172,3 +172,24 @@ def test_stageA_full_config1_with_scatter(gpu, oracle_lib):
     _close(D.to_host(number), n_o, 1e-7, atol=1e-12 * n_o.max(), what="number")
     h_o = o['h2fdf'].reshape(-1, F)
     _close(D.to_host(h2fdf), h_o, 1e-9, what="h2fdf")
+
+
+def test_mass_stellar(gpu, oracle_lib):
+    """mass_stellar (sam.py:250-278) on the device against the oracle's M-Mbulge relation."""
+    print("[mass_stellar config2 small]", flush=True)
+    from oracle.cfg import oracle_cfg
+    from oracle.pipeline import mstar_from_mbh
+    spec = configs.spec_config2(shape=SMALL, nfreq=7)
+    sam = configs.make_sam(spec, scatter=0.0)
+    mp, mr, mt, rz = sam.mass_stellar()
+    mmb = oracle_cfg(spec)['mmbulge']
+    m1 = sam.mtot[:, None] / (1.0 + sam.mrat[None, :])
+    m2 = sam.mtot[:, None] - m1
+    s1 = np.broadcast_to(mstar_from_mbh(m1, mmb)[..., None], mp.shape)
+    s2 = np.broadcast_to(mstar_from_mbh(m2, mmb)[..., None], mp.shape)
+    _close(mp, s1, 1e-12, what="mstar_pri")
+    _close(mr, s2 / s1, 1e-12, what="mstar_rat")
+    _close(mt, s1 + s2, 1e-12, what="mstar_tot")
+    assert rz.shape == mp.shape and np.array_equal(rz[0, 0], sam.redz)
+    # secondary never heavier than primary (the relation is monotonic and q <= 1)
+    assert np.all(mr <= 1.0 + 1e-15)
