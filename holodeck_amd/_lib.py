@@ -53,6 +53,10 @@ SIGNATURES = {
     "hd_realize": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _D, C.c_uint64, _P, _I32, _P, _P, _P,
                           _P, _P, _SZ, _P]),
     "hd_philox4x32_10": (None, [_P, _P, _P]),
+    "hd_xoshiro128ss": (None, [_P, _I64, _P]),
+    "hd_poisson_cdf_table": (_I32, [C.c_double, _I32, _P, _P, _P]),
+    "hd_table_draw_words": (_I32, [_P, _I32, _P, _I64, _P, _P, _P]),
+    "hd_table_draw_pair_words": (_I32, [_P, _I32, _P, _I32, _P, _I64, _P, _P, _P]),
     "hd_mlp_gemm_bf16": (_I32, [_P, _I64, _I64, _P, _I64, _I64, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _I64, _P]),
     "hd_mlp_colsum_bf16": (_I32, [_P, _I32, _I32, _P, _P]),
     "hd_mlp_mse": (_I32, [_P, _P, _I64, _P, _P, _P]),
@@ -141,3 +145,13 @@ def timing_read(name):
     n = C.c_int(0)
     call("hd_timing_read", name.encode(), C.addressof(ms), C.addressof(n))
     return ms.value, n.value
+
+
+def xoshiro128ss(state, n):
+    """Host xoshiro128**: (state after n steps, the n outputs as uint32)."""
+    import numpy as np
+    lib = load()
+    s = (C.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in state])
+    out = np.zeros(max(int(n), 0), dtype=np.uint32)
+    lib.hd_xoshiro128ss(C.addressof(s), int(n), out.ctypes.data if n > 0 else None)
+    return [int(v) for v in s], out

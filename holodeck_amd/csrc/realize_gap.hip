@@ -23,7 +23,7 @@
 //         P(X <= lambda - x) <= exp(-x^2 / 2 lambda))
 //   khi = lambda + b/3 + sqrt(b^2/9 + 2 b lambda) (Bernstein upper tail:
 //         P(X >= lambda + x) <= exp(-x^2 / (2 (lambda + x/3))))
-// so each dropped tail holds less than 2^-56, below the 2^-53 resolution of the
+// so each dropped tail holds less than 2^-56, below the 2^-64 resolution of the
 // uniforms; a uniform above the table's last value (CDF rounding, ~n ulp)
 // returns khi.  Bins whose window does not fit the LDS table, and lambda above
 // the normal threshold, are drawn by k_gz_big afterwards (PTRS, Hoermann 1993;
@@ -31,6 +31,11 @@
 // continues the same partials.  The sparse tile total N uses the same CDF
 // machinery (as a pseudo-record appended to the tile; its lambda is below
 // 512 tau, so its window always fits).
+//
+// Random numbers: each (tile, frequency, realization) has one xoshiro128**
+// stream seeded by Philox4x32-10 of (tile, f, r, tag 6) under the seed; the
+// tile's records take its words in order (gz_table_draw32), so results do not
+// depend on launch shape or batching.
 //
 // Three kernels:
 //   k_gz_prep  -- per (tile of 512 bins, frequency): classify, compact into
@@ -128,25 +133,43 @@ __device__ __forceinline__ void gz_window(double lam, int cap, int& klo, int& n)
     }
 }
 
-// log(k!) for k = 0..15 (exact to double rounding)
-__constant__ double gz_logfact[16] = {
-    0.0, 0.0, 0.69314718055994530942, 1.79175946922805500081, 3.17805383034794561964, 4.78749174278204599424,
-    6.57925121201010099506, 8.52516136106541430017, 10.60460290274525022842, 12.80182748008146961121,
-    15.10441257307551529523, 17.50230784587388583929, 19.98721449566188614952, 22.55216385312342288557,
-    25.19122118273868150009, 27.89927138384089156609};
+// Stirling-formula error ln(k!) - (k + 1/2) ln k + k - ln sqrt(2 pi): k = 1..15 (from
+// the definition at 40 digits), series above (error < 2e-16 at k = 16).
+__constant__ double gz_stirlerr[16] = {
+    0.0, 0.081061466795327258220, 0.041340695955409294094, 0.027677925684998339149, 0.020790672103765093112,
+    0.016644691189821192163, 0.013876128823070747999, 0.011896709945891770095, 0.010411265261972096497,
+    0.0092554621827127329177, 0.0083305634333628712565, 0.0075736754879518407950, 0.0069428401072095298657,
+    0.0064089941880042070684, 0.0059513701127588477356, 0.0055547335519628013710};
 
-// log(k!) for integer k >= 0: table below 16, Stirling series in x = k+1 above
-// (truncation error of the omitted x^-9 term < 1e-14 absolute at x = 17).
-__device__ __forceinline__ double gz_logfact_k(double k) {
-    if (k < 16.0) return gz_logfact[(int)k];
-    const double x = k + 1.0;
-    const double r = 1.0 / x, r2 = r * r;
-    const double series = r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0 - r2 * (1.0 / 1680.0))));
-    return (x - 0.5) * log(x) - x + 0.91893853320467274178 + series;
+__device__ __forceinline__ double gz_stirling_err(double k) {
+    if (k < 16.0) return gz_stirlerr[(int)k];
+    const double r = 1.0 / k, r2 = r * r;
+    return r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0 - r2 * (1.0 / 1680.0 - r2 * (1.0 / 1188.0)))));
 }
 
-__device__ __forceinline__ double gz_pmf(double lam, double loglam, double k) {
-    return exp(k * loglam - lam - gz_logfact_k(k));
+// k ln(k / lam) + lam - k without cancellation: the series in v = (k - lam) / (k + lam)
+// near k = lam (C. Loader, "Fast and accurate computation of binomial probabilities", 2000).
+__device__ __forceinline__ double gz_bd0(double k, double lam) {
+    if (fabs(k - lam) < 0.1 * (k + lam)) {
+        const double v = (k - lam) / (k + lam), v2 = v * v;
+        double s = (k - lam) * v, ej = 2.0 * k * v;
+        for (int j = 1; j < 40; j++) {
+            ej *= v2;
+            const double s1 = s + ej / (double)(2 * j + 1);
+            if (s1 == s) break;
+            s = s1;
+        }
+        return s;
+    }
+    return k * log(k / lam) + lam - k;
+}
+
+// Poisson pmf with relative error ~1e-15 at any lam (the saddle-point form of the same
+// paper; exp(k ln lam - lam - ln k!) would lose ~|k ln lam| ulp to cancellation, 1e-10 at
+// lam ~ 1e5).
+__device__ __forceinline__ double gz_pmf_acc(double lam, double k) {
+    if (k == 0.0) return exp(-lam);
+    return exp(-gz_stirling_err(k) - gz_bd0(k, lam)) * rsqrt(2.0 * M_PI * k);
 }
 
 // ---------------------------------------------------------------------------
@@ -231,7 +254,7 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
         r.word = word;
         r.klo = klo;
         r.n = nwin;
-        r.p0 = (nwin > 0) ? gz_pmf(lam, log(lam), (double)klo) : 0.0;
+        r.p0 = (nwin > 0) ? gz_pmf_acc(lam, (double)klo) : 0.0;
         r.off = on + xn - nwin;   // exclusive prefix of the window lengths
         r.pad = 0;
         A.drec[rreg + i] = r;
@@ -253,7 +276,7 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
             r.h = 0.0;
             r.word = 0;
             gz_window(cum, A.cap, r.klo, r.n);
-            r.p0 = (r.n > 0) ? gz_pmf(cum, log(cum), (double)r.klo) : 0.0;
+            r.p0 = (r.n > 0) ? gz_pmf_acc(cum, (double)r.klo) : 0.0;
             r.off = ntot;
             r.pad = 0;
             A.drec[rreg + nd] = r;
@@ -415,13 +438,29 @@ __device__ __forceinline__ double gz_rcp(double x) {
     return fma(r, e, r);
 }
 
-// Build the CDF window of one record into tab[0, n) with one wave.  Each lane
-// takes a contiguous segment; p(klo) = p0 comes from the prep kernel, and the
-// pmf along the window follows p(k+1) = p(k) lam / (k+1): the lanes' segment
-// products of these ratios are joined by a wave prefix product, then each lane
-// runs its segment's recursion and partial sums, which a wave prefix sum joins.
-// (Relative error of the tables about n ulp, ~1e-12 for the widest window.)
-__device__ __noinline__ void gz_build(double* tab, double lam, int klo, int n, double p0, int lane) {
+// CDF value c in [0, 1] as a 64-bit fixed-point fraction floor(c 2^64) split
+// into its high and low 32-bit words (c >= 1 saturates); exact.
+__device__ __forceinline__ void gz_fixed64(double c, uint32_t& h, uint32_t& l) {
+    if (!(c < 1.0)) {
+        h = l = 0xFFFFFFFFu;
+        return;
+    }
+    const double t = c * 4294967296.0;   // exact
+    h = (uint32_t)t;
+    l = (uint32_t)((t - (double)h) * 4294967296.0);   // the fraction is exact, so is its scaling
+}
+
+// Build the CDF window of one record with one wave, as 64-bit fixed point:
+// hi[0, n) the high words, hi[n, 2n) the low words.  Each lane takes a
+// contiguous segment; p(klo) = p0 comes from the prep kernel, and the pmf
+// along the window follows p(k+1) = p(k) lam / (k+1): the lanes' segment
+// products of these ratios are joined by a wave prefix product, each lane
+// sums its segment's recursion, a wave prefix sum gives the carries, and a
+// second run of the recursion writes carry + partial sums.  (Relative error
+// of the CDF about n ulp, ~1e-12 for the widest window, then rounding down to
+// a multiple of 2^-64.)
+__device__ __noinline__ void gz_build(uint32_t* hi, double lam, int klo, int n, double p0, int lane) {
+    uint32_t* lo = hi + n;
     const int s = (n + 63) >> 6;
     const int j0 = lane * s;
     const int m = min(max(n - j0, 0), s);
@@ -436,12 +475,12 @@ __device__ __noinline__ void gz_build(double* tab, double lam, int klo, int n, d
     }
     double pre = __shfl_up(px, 1, 64);   // product over the lanes before this one
     if (lane == 0) pre = 1.0;
+    const double pstart = p0 * pre;      // p(klo + j0)
     double local = 0.0;
-    if (m > 0) {
-        double p = p0 * pre;                   // p(klo + j0)
+    {
+        double p = pstart;
         for (int j = 0; j < m; j++) {
             local += p;
-            tab[j0 + j] = local;
             p *= lam * gz_rcp(kb + (double)(j + 1));
         }
     }
@@ -451,13 +490,8 @@ __device__ __noinline__ void gz_build(double* tab, double lam, int klo, int n, d
         const double y = __shfl_up(x, off, 64);
         if (lane >= off) x += y;
     }
-    const double carry = x - local;
-    double last = -1.0;
-    if (m > 0) {
-        if (lane > 0)
-            for (int j = 0; j < m; j++) tab[j0 + j] += carry;
-        last = tab[j0 + m - 1];
-    }
+    const double carry = (lane > 0) ? x - local : 0.0;
+    const double last = (m > 0) ? local + carry : -1.0;
     // The carries are rounded independently of the segment sums, so two
     // neighbouring segments can meet out of order by an ulp; raise each
     // segment to the largest value before it, which makes the table exactly
@@ -468,33 +502,90 @@ __device__ __noinline__ void gz_build(double* tab, double lam, int klo, int n, d
         const double y = __shfl_up(mx, off, 64);
         if (lane >= off) mx = fmax(mx, y);
     }
-    const double before = __shfl_up(mx, 1, 64);
-    if (m > 0 && lane > 0 && tab[j0] < before)
-        for (int j = 0; j < m; j++) tab[j0 + j] = fmax(tab[j0 + j], before);
-}
-
-// Uniform in [0, 1) from 52 random bits: a double in [1, 2) minus 1 (exact).
-__device__ __forceinline__ double gz_u52(uint32_t hi, uint32_t lo) {
-    const uint64_t bits = 0x3FF0000000000000ull | ((uint64_t)(hi & 0xFFFFFu) << 32) | lo;
-    return __longlong_as_double((long long)bits) - 1.0;
-}
-
-// Inversion: the smallest k in the window with CDF(k) > u (branch-free binary
-// search); a uniform above the table's last value (CDF rounding near 1)
-// returns the window's end.
-// (The candidate is carried as an address, so a step costs one add, one compare and one
-// select.)
-__device__ __forceinline__ double gz_table_draw(const double* tab, int klo, int n, double u) {
-    const double* p = tab;   // invariant: the answer is in [p - tab, p - tab + len]
-    int len = n;
-    while (len > 1) {
-        const int half = len >> 1;
-        const double* q = p + half;
-        p = (*q <= u) ? q : p;
-        len -= half;
+    double before = __shfl_up(mx, 1, 64);
+    if (lane == 0) before = 0.0;
+    double p = pstart, acc = 0.0;
+    for (int j = 0; j < m; j++) {
+        acc += p;
+        uint32_t h, l;
+        gz_fixed64(fmax(acc + carry, before), h, l);
+        hi[j0 + j] = h;
+        lo[j0 + j] = l;
+        p *= lam * gz_rcp(kb + (double)(j + 1));
     }
-    const int base = (int)(p - tab);
-    return (double)(klo + min(base + (*p <= u ? 1 : 0), n - 1));
+}
+
+// Inversion by the stream: the smallest k in the window with CDF(k) > u,
+// u = (x, w) / 2^64 for the next words x, w.  A branch-free binary search over
+// the high words finds the first k with hi[k] >= x; when that high word equals
+// x the low words decide, so only then is w taken (probability about n 2^-32
+// per draw).  A uniform above the table's last value (CDF rounding near 1)
+// returns the window's end.  The result is the window index (0 .. n-1);
+// hi[n] (the first low word) is read but not used.  (The candidate is carried
+// as an address, so a step costs one add, one compare and one select, on
+// 4-byte words.)  Two searches run interleaved (gz_table_draw32x2) so their
+// LDS round trips overlap.
+struct GzSearch {
+    const uint32_t* hi;
+    const uint32_t* p;   // invariant: the first k with hi[k] >= x is in [p - hi, p - hi + len]
+    int n, len;
+    uint32_t x;
+};
+
+__device__ __forceinline__ void gz_search_start(GzSearch& s, const uint32_t* hi, int n, uint32_t x) {
+    s.hi = hi;
+    s.p = hi;
+    s.n = n;
+    s.len = n;
+    s.x = x;
+}
+
+__device__ __forceinline__ void gz_search_step(GzSearch& s) {
+    const int half = s.len >> 1;
+    const uint32_t* q = s.p + half;
+    s.p = (*q < s.x) ? q : s.p;
+    s.len -= half;
+}
+
+template <class Words>
+__device__ __forceinline__ int gz_search_finish(const GzSearch& s, Words& rs) {
+    const uint32_t h0 = s.p[0], h1 = s.p[1];
+    const bool up = h0 < s.x;
+    int k = (int)(s.p - s.hi) + (up ? 1 : 0);
+    if (k < s.n && (up ? h1 : h0) == s.x) {
+        const uint32_t w = rs.next();
+        const uint32_t* lo = s.hi + s.n;
+        while (k < s.n - 1 && s.hi[k] == s.x && lo[k] <= w) k++;
+    }
+    return min(k, s.n - 1);
+}
+
+template <class Words>
+__device__ __forceinline__ int gz_table_draw32(const uint32_t* hi, int n, Words& rs) {
+    GzSearch s;
+    gz_search_start(s, hi, n, rs.next());
+    while (s.len > 1) gz_search_step(s);
+    return gz_search_finish(s, rs);
+}
+
+// Two draws from tables a and b (n = 0: no draw).  Words in the order x_a, x_b,
+// then w_a, w_b where needed.
+template <class Words>
+__device__ __forceinline__ void gz_table_draw32x2(const uint32_t* ha, int na, const uint32_t* hb, int nb, Words& rs,
+                                                  int& ka, int& kb) {
+    GzSearch a, b;
+    const uint32_t xa = (na > 0) ? rs.next() : 0u;
+    const uint32_t xb = (nb > 0) ? rs.next() : 0u;
+    gz_search_start(a, ha, na, xa);
+    gz_search_start(b, hb, nb, xb);
+    while (a.len > 1 && b.len > 1) {
+        gz_search_step(a);
+        gz_search_step(b);
+    }
+    while (a.len > 1) gz_search_step(a);
+    while (b.len > 1) gz_search_step(b);
+    ka = (na > 0) ? gz_search_finish(a, rs) : 0;
+    kb = (nb > 0) ? gz_search_finish(b, rs) : 0;
 }
 
 // Counts without a table: Normal above the threshold, inversion below 10,
@@ -530,7 +621,7 @@ template <int MODE, int LM, int NP>
 __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
     extern __shared__ double gz_lds[];
     __shared__ int s_next;             // next record of the batch to build
-    double* tab = gz_lds;              // A.cap doubles
+    uint32_t* tab = (uint32_t*)gz_lds;   // A.cap 8-byte slots: per record n high words, then n low words
     double* scum = gz_lds + A.cap;     // GZ_TILE doubles
     BRec* brec = (BRec*)(scum + GZ_TILE);   // GZ_BATCH_REC staged record headers
     const int f = blockIdx.y;
@@ -555,6 +646,10 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
             const int nd = A.cnt[ft * 3];
             const int ns = A.cnt[ft * 3 + 1];
             const int nrec = nd + (ns > 0 ? 1 : 0);
+            // the realization's stream for this tile: records in order take one word each
+            // (two when the 32-bit draw is open), then the sparse points two words each
+            Xo128 rs;
+            rs.seed(philox4x32_10(U4{(uint32_t)tile, (uint32_t)f, (uint32_t)r, 6u << 16}, k0, k1));
             int c = 0;
             while (c < nrec) {
                 // batch [c, c1): consecutive records whose windows fit the table
@@ -587,7 +682,7 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                         DRec nrec;
                         if (inext < c1) nrec = A.drec[rreg + inext];
                         if (lane == 0) brec[i - c] = BRec{rec.h, rec.word, rec.klo, rec.n, rec.off - base, 0};
-                        if (rec.n > 0) gz_build(tab + (rec.off - base), rec.lam, rec.klo, rec.n, rec.p0, lane);
+                        if (rec.n > 0) gz_build(tab + 2 * (rec.off - base), rec.lam, rec.klo, rec.n, rec.p0, lane);
                         i = inext;
                         rec = nrec;
                     }
@@ -596,38 +691,32 @@ __global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
                     for (int j = tid; j < ns; j += bs) scum[j] = A.s_cum[reg + j];
                 __syncthreads();
                 if (active) {
-                    // One Philox block per record pair (2i, 2i+1) of the tile: words (x, y) for the
-                    // even record, (z, w) for the odd one; counter (tile, f, r, tag 5 | pair).
-                    int blk_pair = -1;
-                    U4 blk = {0u, 0u, 0u, 0u};
                     // dense records of the batch (one control path, so the running list needs no
                     // copies between branches), then the tile's sparse pseudo-record if it is here
+                    // (records in pairs, their searches interleaved; n = 0: no table, drawn by k_gz_big)
                     const int cd = min(c1, nd);
-                    for (int i = c; i < cd; i++) {
-                        const BRec rec = brec[i - c];
-                        if (rec.n == 0) continue;   // no table: drawn by k_gz_big
-                        if ((i >> 1) != blk_pair) {
-                            blk_pair = i >> 1;
-                            blk = philox4x32_10(U4{(uint32_t)tile, (uint32_t)f, (uint32_t)r,
-                                                   (5u << 16) | (uint32_t)blk_pair}, k0, k1);
-                        }
-                        const double u = (i & 1) ? gz_u52(blk.z, blk.w) : gz_u52(blk.x, blk.y);
-                        const double n = gz_table_draw(tab + rec.off, rec.klo, rec.n, u);
-                        acc.add(n, rec.h, rec.word, A.d_par + reg + i, pstride, A.npar, L);
+                    for (int i = c; i < cd; i += 2) {
+                        const BRec ra = brec[i - c];
+                        const bool hb = i + 1 < cd;
+                        const BRec rb = brec[hb ? i + 1 - c : i - c];
+                        const int na = ra.n, nb = hb ? rb.n : 0;
+                        int ka, kb;
+                        gz_table_draw32x2(tab + 2 * ra.off, na, tab + 2 * rb.off, nb, rs, ka, kb);
+                        if (na > 0) acc.add((double)(ra.klo + ka), ra.h, ra.word, A.d_par + reg + i, pstride, A.npar, L);
+                        if (nb > 0)
+                            acc.add((double)(rb.klo + kb), rb.h, rb.word, A.d_par + reg + i + 1, pstride, A.npar, L);
                     }
                     for (int i = max(c, nd); i < c1; i++) {
                         const BRec rec = brec[i - c];
                         {
                             // sparse bins: Poisson process on the tile's cumulative-lambda axis; the
-                            // point count from stream tag 4 (its window always fits: lambda < 512),
-                            // the positions from stream tag 2
+                            // point count from its table (the window always fits: lambda < 512), the
+                            // positions uniform on the axis (53-bit uniforms)
                             const double Lam = scum[ns - 1];
-                            const U4 a = philox4x32_10(U4{(uint32_t)tile, (uint32_t)f, (uint32_t)r, 4u << 16}, k0, k1);
-                            const int np = (int)gz_table_draw(tab + rec.off, rec.klo, rec.n, gz_u52(a.x, a.y));
-                            Stream st;
-                            st.init((uint32_t)tile, (uint32_t)f, (uint32_t)r, 2u, A.seed);
+                            const int np = rec.klo + gz_table_draw32(tab + 2 * rec.off, rec.n, rs);
                             for (int q = 0; q < np; q++) {
-                                const double xq = st.next() * Lam;
+                                const uint32_t w0 = rs.next();
+                                const double xq = u53(w0, rs.next()) * Lam;
                                 int lo = 0, hi = ns - 1;
                                 while (lo < hi) {   // first j with scum[j] > xq
                                     const int mid = (lo + hi) >> 1;
@@ -684,6 +773,50 @@ __global__ void __launch_bounds__(256) k_gz_big(GapArgs A) {
         }
         gz_store<MODE, LM, NP>(A, acc, pr);
     }
+}
+
+// ---------------------------------------------------------------------------
+// hooks for tests: one CDF window built by one wave; draws from given words
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_cdf_table_hook(double lam, int cap, int* kn, uint64_t* cdf_out) {
+    extern __shared__ double hook_lds[];
+    uint32_t* tab = (uint32_t*)hook_lds;
+    const int lane = threadIdx.x;
+    int klo, n;
+    gz_window(lam, cap, klo, n);
+    if (n > 0) gz_build(tab, lam, klo, n, gz_pmf_acc(lam, (double)klo), lane);
+    __syncthreads();
+    for (int j = lane; j < n; j += 64) cdf_out[j] = ((uint64_t)tab[j] << 32) | tab[n + j];
+    if (lane == 0) {
+        kn[0] = klo;
+        kn[1] = n;
+    }
+}
+
+struct GzWords {   // a stream that replays given words
+    const uint32_t* w;
+    int i;
+    __device__ uint32_t next() { return w[i++]; }
+};
+
+__global__ void k_table_draw_hook(const uint32_t* tab, int n, const uint32_t* words, int64_t m, int* slot, int* used) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    GzWords ws{words + 2 * i, 0};
+    slot[i] = gz_table_draw32(tab, n, ws);
+    used[i] = ws.i;
+}
+
+__global__ void k_table_draw_pair_hook(const uint32_t* tab, int n, const uint32_t* tab2, int n2, const uint32_t* words,
+                                       int64_t m, int* slot, int* used) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    GzWords ws{words + 4 * i, 0};
+    int ka, kb;
+    gz_table_draw32x2(tab, n, tab2, n2, ws, ka, kb);
+    slot[2 * i] = ka;
+    slot[2 * i + 1] = kb;
+    used[i] = ws.i;
 }
 
 int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int F, int R, int L, int mode, int npar,
@@ -832,4 +965,48 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     if (rc) return rc;
     return realize_merge_launch(h2fdf, par, B, F, R, L, mode, npar, A.nchunks, A.p_total, A.p_ptot, A.p_key, A.p_bin,
                                 hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, st);
+}
+
+extern "C" int hd_poisson_cdf_table(double lam, int cap, int* klo_n, uint64_t* cdf, void* stream) {
+    if (!(lam > 0.0) || !(lam < 1e300) || cap < 1 || cap > 8000 || !klo_n || !cdf)
+        return set_error(HD_ERR_ARG, "hd_poisson_cdf_table: bad arguments");
+    hipLaunchKernelGGL(k_cdf_table_hook, dim3(1), dim3(64), (size_t)cap * sizeof(double), (hipStream_t)stream, lam,
+                       cap, klo_n, cdf);
+    return check_launch("k_cdf_table_hook");
+}
+
+extern "C" int hd_table_draw_words(const uint32_t* tab, int n, const uint32_t* words, int64_t m, int* slot, int* used,
+                                   void* stream) {
+    if (n < 1 || m < 0 || !tab || (m > 0 && (!words || !slot || !used)))
+        return set_error(HD_ERR_ARG, "hd_table_draw_words: bad arguments");
+    if (m == 0) return HD_OK;
+    hipLaunchKernelGGL(k_table_draw_hook, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, (hipStream_t)stream, tab, n,
+                       words, m, slot, used);
+    return check_launch("k_table_draw_hook");
+}
+
+extern "C" void hd_xoshiro128ss(uint32_t* state, int64_t n, uint32_t* out) {
+    Xo128 g;
+    g.s0 = state[0];
+    g.s1 = state[1];
+    g.s2 = state[2];
+    g.s3 = state[3];
+    for (int64_t i = 0; i < n; i++) {
+        const uint32_t v = g.next();
+        if (out) out[i] = v;
+    }
+    state[0] = g.s0;
+    state[1] = g.s1;
+    state[2] = g.s2;
+    state[3] = g.s3;
+}
+
+extern "C" int hd_table_draw_pair_words(const uint32_t* tab, int n, const uint32_t* tab2, int n2,
+                                        const uint32_t* words, int64_t m, int* slot, int* used, void* stream) {
+    if (n < 1 || n2 < 1 || m < 0 || !tab || !tab2 || (m > 0 && (!words || !slot || !used)))
+        return set_error(HD_ERR_ARG, "hd_table_draw_pair_words: bad arguments");
+    if (m == 0) return HD_OK;
+    hipLaunchKernelGGL(k_table_draw_pair_hook, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, (hipStream_t)stream, tab,
+                       n, tab2, n2, words, m, slot, used);
+    return check_launch("k_table_draw_pair_hook");
 }

@@ -33,6 +33,32 @@ __host__ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t
     return c;
 }
 
+// xoshiro128** (Blackman & Vigna, "Scrambled linear pseudorandom number
+// generators", ACM TOMS 47, 2021): a 128-bit xorshift state with the "**"
+// output scrambler, 32 bits per step, all full-rate integer ops.  Used as a
+// short sequential stream seeded from one Philox block (a keyed hash of the
+// stream's coordinates), so streams stay independent and reproducible while a
+// draw costs a few instructions instead of a Philox evaluation.
+struct Xo128 {
+    uint32_t s0, s1, s2, s3;
+    __host__ __device__ __forceinline__ void seed(U4 w) {
+        s0 = w.x; s1 = w.y; s2 = w.z; s3 = w.w;
+        if ((s0 | s1 | s2 | s3) == 0u) s0 = 0x9E3779B9u;   // the all-zero state is a fixed point
+    }
+    __host__ __device__ __forceinline__ uint32_t next() {
+        const uint32_t m = s1 * 5u;
+        const uint32_t r = ((m << 7) | (m >> 25)) * 9u;
+        const uint32_t t = s1 << 9;
+        s2 ^= s0;
+        s3 ^= s1;
+        s1 ^= s2;
+        s0 ^= s3;
+        s2 ^= t;
+        s3 = (s3 << 11) | (s3 >> 21);
+        return r;
+    }
+};
+
 // uniform in (0,1) with 53 random bits
 __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
     uint64_t x = ((uint64_t)hi << 21) | (lo >> 11);

@@ -216,6 +216,22 @@ int hd_realize(const double *number, const double *h2fdf, const int64_t *order, 
 /* Philox4x32-10 block function (host), for known-answer tests of the
  * generator the realization kernels use. */
 void hd_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+/* xoshiro128** (host): advances state[4] n times, writing the outputs to out (may be NULL).
+ * The realization kernels draw from one such stream per (tile, frequency, realization), seeded
+ * by Philox; exported for tests of the generator. */
+void hd_xoshiro128ss(uint32_t *state, int64_t n, uint32_t *out);
+/* Test hooks of the realization kernels' sampler (no counterpart in the reference, whose
+ * numpy Poisson draws they replace): the CDF table of the Poisson(lam) window built by one wave
+ * (klo_n = {klo, n}; cdf[n] = floor(CDF(klo + k) 2^64), saturating; cap at most 8000), and
+ * inversions of such a table with given words (tab = the n high words then the n low words;
+ * two words per draw; slot[i] the window index drawn, used[i] the words it took). */
+int hd_poisson_cdf_table(double lam, int cap, int *klo_n, uint64_t *cdf, void *stream);
+int hd_table_draw_words(const uint32_t *tab, int n, const uint32_t *words, int64_t m, int *slot, int *used,
+                        void *stream);
+/* The same for two tables drawn together (the kernels' interleaved pair): four words per pair,
+ * taken as x_a, x_b, then w_a and w_b where needed; slot[2i], slot[2i+1] the indices drawn. */
+int hd_table_draw_pair_words(const uint32_t *tab, int n, const uint32_t *tab2, int n2, const uint32_t *words,
+                             int64_t m, int *slot, int *used, void *stream);
 
 /* ------------------------------------------------------------------------
  * Learned surrogate (MLP: parameters -> log10 h_c percentiles).  No
