@@ -61,7 +61,7 @@ constexpr int GZ_TILE = 512;          // bins per tile (prep workgroup size)
 constexpr int GZ_REC = GZ_TILE + 1;   // record slots per tile (+1 sparse pseudo-record)
 constexpr int GZ_TILES_PER_CHUNK = 16;
 constexpr int GZ_MAXBS = 512;         // draw workgroup: one realization per thread
-constexpr int GZ_BATCH_REC = 256;     // at most this many records per LDS batch (their headers are staged)
+constexpr int GZ_BATCH_REC = 128;     // at most this many records per LDS batch (their headers are staged)
 constexpr int GZ_XCDS = 8;            // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr uint64_t GZ_KEY_NONE = ~0ull;
 constexpr double GZ_TAILB = 38.816242111356935;   // 56 ln 2: each dropped tail < 2^-56
@@ -618,7 +618,8 @@ __device__ __forceinline__ void gz_store(const GapArgs& A, const Acc<MODE, LM, N
 }
 
 template <int MODE, int LM, int NP>
-__global__ void __launch_bounds__(GZ_MAXBS) k_gz_draw(GapArgs A) {
+// (6 waves per SIMD: with the LDS below three 512-thread workgroups share a CU)
+__global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
     extern __shared__ double gz_lds[];
     __shared__ int s_next;             // next record of the batch to build
     uint32_t* tab = (uint32_t*)gz_lds;   // A.cap 8-byte slots: per record n high words, then n low words
@@ -828,11 +829,13 @@ int rank_from_order_launch(const int64_t* order, int64_t B, int32_t* rank, hipSt
 static size_t gz_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int gz_block(int R) { return (int)min((int64_t)GZ_MAXBS, (int64_t)cdiv(R, 64) * 64); }
-// LDS table capacity (doubles) of every draw workgroup: table + sparse axis +
-// staged record headers + the build counter stay within 64 KiB.  Fixed, so that which bins get
-// a table (and so every draw) does not depend on the launch shape.
+// LDS table capacity (8-byte slots) of every draw workgroup: table + sparse
+// axis + staged record headers + the build counter stay within 53 KB, so that
+// three workgroups fit one CU's 160 KB (allocation granularity included).
+// Fixed, so that which bins get a table (and so every draw) does not depend
+// on the launch shape.
 #ifndef HD_GZ_CAP
-#define HD_GZ_CAP 6392
+#define HD_GZ_CAP 5632
 #endif
 static int gz_cap(int) { return HD_GZ_CAP; }
 
