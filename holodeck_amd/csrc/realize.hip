@@ -161,26 +161,12 @@ __global__ void __launch_bounds__(512) k_realize_chunk(RealizeArgs A) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_realize_merge(RealizeArgs A, double* __restrict__ hc2ss,
-                                                       int64_t* __restrict__ ssidx, double* __restrict__ hc2bg,
-                                                       double* __restrict__ bgpar) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Merge of chunk partials [c0, c1) of (f, r) into a running list, total and parameter totals.
+__device__ __forceinline__ void merge_chunks(const RealizeArgs& A, int f, int r, int c0, int c1, uint64_t (&lk)[LMAX],
+                                             int32_t (&lb)[LMAX], double& total, double (&ptot)[NPAR_MAX]) {
     const int F = A.F, R = A.R, L = A.L, NC = A.nchunks;
-    if (t >= (int64_t)F * R) return;
-    const int f = (int)(t / R), r = (int)(t % R);
     const bool use_list = (A.mode != HD_SEL_NONE);
-    uint64_t lk[LMAX];
-    int32_t lb[LMAX];
-#pragma unroll
-    for (int i = 0; i < LMAX; i++) {
-        lk[i] = KEY_NONE;
-        lb[i] = 0x7fffffff;
-    }
-    double total = 0.0;
-    double ptot[NPAR_MAX];
-#pragma unroll
-    for (int k = 0; k < NPAR_MAX; k++) ptot[k] = 0.0;
-    for (int c = 0; c < NC; c++) {
+    for (int c = c0; c < c1; c++) {
         int64_t pr = ((int64_t)f * NC + c) * R + r;
         total += A.p_total[pr];
         for (int k = 0; k < A.npar; k++) ptot[k] += A.p_ptot[(int64_t)k * F * NC * R + pr];
@@ -200,6 +186,68 @@ __global__ void __launch_bounds__(256) k_realize_merge(RealizeArgs A, double* __
             }
         }
     }
+}
+
+constexpr int MERGE_GROUPS = 8;
+
+// First merge stage: thread per (f, group, r) merges the group's chunks and writes a partial of
+// the same layout with G "chunks", so that the final merge reads G partials instead of NC (the
+// final merge alone has one thread per (f, r): too few waves to hide its load latency).
+__global__ void __launch_bounds__(256) k_realize_group(RealizeArgs A, int G, double* __restrict__ q_total,
+                                                       double* __restrict__ q_ptot, uint64_t* __restrict__ q_key,
+                                                       int32_t* __restrict__ q_bin) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = A.F, R = A.R, L = A.L, NC = A.nchunks;
+    if (t >= (int64_t)F * G * R) return;
+    const int r = (int)(t % R), g = (int)((t / R) % G), f = (int)(t / ((int64_t)G * R));
+    const int CG = (NC + G - 1) / G;
+    const int c0 = min(g * CG, NC), c1 = min(c0 + CG, NC);
+    uint64_t lk[LMAX];
+    int32_t lb[LMAX];
+#pragma unroll
+    for (int i = 0; i < LMAX; i++) {
+        lk[i] = KEY_NONE;
+        lb[i] = 0x7fffffff;
+    }
+    double total = 0.0;
+    double ptot[NPAR_MAX];
+#pragma unroll
+    for (int k = 0; k < NPAR_MAX; k++) ptot[k] = 0.0;
+    merge_chunks(A, f, r, c0, c1, lk, lb, total, ptot);
+    const int64_t pr = ((int64_t)f * G + g) * R + r;
+    q_total[pr] = total;
+    for (int k = 0; k < A.npar; k++) q_ptot[(int64_t)k * F * G * R + pr] = ptot[k];
+    if (A.mode != HD_SEL_NONE) {
+#pragma unroll
+        for (int j = 0; j < LMAX; j++)
+            if (j < L) {
+                const int64_t o = (((int64_t)f * G + g) * L + j) * R + r;
+                q_key[o] = lk[j];
+                q_bin[o] = lb[j];
+            }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_realize_merge(RealizeArgs A, double* __restrict__ hc2ss,
+                                                       int64_t* __restrict__ ssidx, double* __restrict__ hc2bg,
+                                                       double* __restrict__ bgpar) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = A.F, R = A.R, L = A.L, NC = A.nchunks;
+    if (t >= (int64_t)F * R) return;
+    const int f = (int)(t / R), r = (int)(t % R);
+    const bool use_list = (A.mode != HD_SEL_NONE);
+    uint64_t lk[LMAX];
+    int32_t lb[LMAX];
+#pragma unroll
+    for (int i = 0; i < LMAX; i++) {
+        lk[i] = KEY_NONE;
+        lb[i] = 0x7fffffff;
+    }
+    double total = 0.0;
+    double ptot[NPAR_MAX];
+#pragma unroll
+    for (int k = 0; k < NPAR_MAX; k++) ptot[k] = 0.0;
+    merge_chunks(A, f, r, 0, NC, lk, lb, total, ptot);
     double sel = 0.0;
     double psel[NPAR_MAX];
 #pragma unroll
@@ -309,10 +357,15 @@ static int realize_counts_impl(const double* counts, const double* h2fdf, const 
 }
 
 // Merge of per-chunk partials, shared with the drawn-count kernel (realize_gap.hip).
+size_t realize_group_ws(int F, int R, int L, int npar) {
+    return (size_t)F * MERGE_GROUPS * R * (sizeof(double) * (1 + (size_t)npar) + (sizeof(uint64_t) + sizeof(int32_t)) * (size_t)L) +
+           4 * 256;
+}
+
 int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int F, int R, int L, int mode, int npar,
                          int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
                          const int32_t* p_bin, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
-                         hipStream_t st) {
+                         void* group_ws, hipStream_t st) {
     RealizeArgs A;
     A.counts = nullptr;
     A.h2fdf = h2fdf;
@@ -330,6 +383,24 @@ int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int 
     A.p_ptot = const_cast<double*>(p_ptot);
     A.p_key = const_cast<uint64_t*>(p_key);
     A.p_bin = const_cast<int32_t*>(p_bin);
+    if (group_ws && nchunks > 2 * MERGE_GROUPS) {
+        const int G = MERGE_GROUPS;
+        const size_t n = (size_t)F * G * R;
+        char* w = (char*)(((uintptr_t)group_ws + 255) & ~(uintptr_t)255);
+        double* q_total = (double*)w;         w += n * sizeof(double);
+        double* q_ptot = (double*)w;          w += n * sizeof(double) * (size_t)npar;
+        uint64_t* q_key = (uint64_t*)w;       w += n * sizeof(uint64_t) * (size_t)L;
+        int32_t* q_bin = (int32_t*)w;
+        hipLaunchKernelGGL(k_realize_group, dim3((unsigned)cdiv((int64_t)n, 256)), dim3(256), 0, st, A, G, q_total,
+                           q_ptot, q_key, q_bin);
+        int rc = check_launch("k_realize_group");
+        if (rc) return rc;
+        A.nchunks = G;
+        A.p_total = q_total;
+        A.p_ptot = q_ptot;
+        A.p_key = q_key;
+        A.p_bin = q_bin;
+    }
     hipLaunchKernelGGL(k_realize_merge, dim3((unsigned)cdiv((int64_t)F * R, 256)), dim3(256), 0, st, A, hc2ss, ssidx,
                        hc2bg, bgpar);
     return check_launch("k_realize_merge");

@@ -823,7 +823,8 @@ __global__ void k_table_draw_pair_hook(const uint32_t* tab, int n, const uint32_
 int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int F, int R, int L, int mode, int npar,
                          int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
                          const int32_t* p_bin, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
-                         hipStream_t st);
+                         void* group_ws, hipStream_t st);
+size_t realize_group_ws(int F, int R, int L, int npar);
 int rank_from_order_launch(const int64_t* order, int64_t B, int32_t* rank, hipStream_t st);
 
 static size_t gz_align(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -853,6 +854,7 @@ static size_t gz_ws(int64_t B, int F, int R, int L, int npar) {
     s += 2 * gz_align(slots * 8 * (size_t)npar);           // params
     s += gz_align(pr * 8) + gz_align(pr * 8 * (size_t)npar) + gz_align(pr * 8 * (size_t)L) +
          gz_align(pr * 4 * (size_t)L);
+    s += gz_align(realize_group_ws(F, R, L, npar));          // first merge stage
     return s + 512;
 }
 
@@ -939,7 +941,8 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     A.p_total = (double*)p;       p += gz_align(pr * 8);
     A.p_ptot = (double*)p;        p += gz_align(pr * 8 * (size_t)npar);
     A.p_key = (uint64_t*)p;       p += gz_align(pr * 8 * (size_t)L);
-    A.p_bin = (int32_t*)p;
+    A.p_bin = (int32_t*)p;        p += gz_align(pr * 4 * (size_t)L);
+    void* group_ws = p;
     A.rank = (mode == HD_SEL_SORTED) ? rank : nullptr;
     if (mode == HD_SEL_SORTED) {
         int rc = rank_from_order_launch(order, B, rank, st);
@@ -967,7 +970,7 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     }
     if (rc) return rc;
     return realize_merge_launch(h2fdf, par, B, F, R, L, mode, npar, A.nchunks, A.p_total, A.p_ptot, A.p_key, A.p_bin,
-                                hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, st);
+                                hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, group_ws, st);
 }
 
 extern "C" int hd_poisson_cdf_table(double lam, int cap, int* klo_n, uint64_t* cdf, void* stream) {
