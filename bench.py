@@ -276,7 +276,7 @@ def main():
                    "streams": len(streams)},
         "cell_realizations_per_s": B * F * R * value,
         "stages_ms": stages,
-        "roofline": {"bound": "hbm", "kernel": "hd_realize (k_gz_prep + k_gz_draw + k_gz_big + k_realize_merge)",
+        "roofline": {"bound": "hbm", "kernel": "hd_realize (k_rank_from_order + k_gz_prep + k_gz_draw + k_gz_big + k_realize_group + k_realize_merge)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "alg_bytes_per_launch": alg_bytes, "launch_ms": t_real * 1e3,
                      "launches": real_n, "k_gz_draw_ms": draw_ms / max(draw_n, 1),
