@@ -226,6 +226,8 @@ __device__ __forceinline__ int bracket_increasing(int n, double val, const doubl
     return lo;
 }
 
+constexpr int DBN_FOBS_LDS = 1024;   // target frequencies staged in LDS up to this many
+
 __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const double* __restrict__ fobs, int F,
                                                           int sorted_f, double sepa_init, int nsteps,
                                                           const double* __restrict__ hard_norm, double rchar,
@@ -248,6 +250,7 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
     double* s_gdc = s_gz + n_interp;
     double* s_gage = s_gdc + n_interp;
     double* s_agek = s_gage + n_interp;   // (Z)
+    double* s_fobs = s_agek + Z;          // (F) when F <= DBN_FOBS_LDS
 
     const int ii = blockIdx.x / Q;
     const int jj = blockIdx.x % Q;
@@ -263,6 +266,11 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
         s_gdc[i] = gdc_g[i];
         s_gage[i] = gage_g[i];
     }
+    // the target frequencies are searched at every evolution step: from LDS, not global memory
+    const bool fobs_lds = F <= DBN_FOBS_LDS;
+    if (fobs_lds)
+        for (int i = tid; i < F; i += blockDim.x) s_fobs[i] = fobs[i];
+    const double* fo = fobs_lds ? s_fobs : fobs;
     // separation chain (sequential, as sam_cyutils.pyx:611-637)
     if (tid == 0) {
         double x = log10(sepa_init);
@@ -330,13 +338,13 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
                 int lo = 0, hi = F;
                 while (lo < hi) {
                     int mid = (lo + hi) >> 1;
-                    if (fobs[mid] < fl) lo = mid + 1;
+                    if (fo[mid] < fl) lo = mid + 1;
                     else hi = mid;
                 }
                 ff = lo;
             }
             for (; ff < F; ff++) {
-                const double ft = fobs[ff];
+                const double ft = fo[ff];
                 if ((ft < fl) || (fr < ft)) {
                     if (sorted_f) break;   // past the contiguous bracketed range
                     continue;
@@ -370,7 +378,7 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
         if (redz_right < 0.0) redz_right = 0.0;
         const double fl = s_frst[step] / (1.0 + redz_left);
         const double fr = s_frst[step + 1] / (1.0 + redz_right);
-        const double ft = fobs[ff];
+        const double ft = fo[ff];
         const double new_time = interp_between(ft, fl, fr, time_left, time_right);
         const int inew = ww_increasing(idx_left, n_interp, new_time, s_gage);
         const double new_redz = interp_at_index(inew, new_time, s_gage, s_gz);
@@ -467,7 +475,8 @@ extern "C" int hd_dbn_2pwl(const double* fobs_orb, int F, int fobs_sorted, doubl
     if (num_steps > DBN_MAX_STEPS) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: num_steps=%d > %d", num_steps, DBN_MAX_STEPS);
     if ((int64_t)M * Q > 0x7fffffff) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: M*Q too large");
     if ((int64_t)Z * F > 0x7fffffff) return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: Z*F too large");
-    const size_t lds = ((size_t)5 * (num_steps + 1) + 3 * (size_t)n_interp + (size_t)Z) * sizeof(double);
+    const size_t lds = ((size_t)5 * (num_steps + 1) + 3 * (size_t)n_interp + (size_t)Z +
+                        (F <= DBN_FOBS_LDS ? (size_t)F : 0)) * sizeof(double);
     if (lds > 65536)
         return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: step/cosmology tables need %zu B of LDS (> 64 KiB)", lds);
     hipLaunchKernelGGL(k_dbn_2pwl, dim3(M * Q), dim3(DBN_THREADS), lds, (hipStream_t)stream, evo_consts(), fobs_orb, F,
