@@ -450,6 +450,39 @@ __device__ __forceinline__ void gz_fixed64(double c, uint32_t& h, uint32_t& l) {
     l = (uint32_t)((t - (double)h) * 4294967296.0);   // the fraction is exact, so is its scaling
 }
 
+// Wave-wide inclusive scans of doubles by DPP moves (row shifts by 1, 2, 4, 8
+// within each 16-lane row, then the broadcasts of lane 15 and lane 31 to the
+// rows after them), in registers instead of LDS permutes.
+template <int CTRL>
+__device__ __forceinline__ double gz_dpp(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <class Op>
+__device__ __forceinline__ double gz_wave_scan(double v, int lane, Op op) {
+    const int rl = lane & 15;
+    double t;
+    t = gz_dpp<0x111>(v);
+    if (rl >= 1) v = op(t, v);
+    t = gz_dpp<0x112>(v);
+    if (rl >= 2) v = op(t, v);
+    t = gz_dpp<0x114>(v);
+    if (rl >= 4) v = op(t, v);
+    t = gz_dpp<0x118>(v);
+    if (rl >= 8) v = op(t, v);
+    t = gz_dpp<0x142>(v);   // row_bcast:15
+    if ((lane & 31) >= 16) v = op(t, v);
+    t = gz_dpp<0x143>(v);   // row_bcast:31
+    if (lane >= 32) v = op(t, v);
+    return v;
+}
+
+struct GzMul { __device__ double operator()(double a, double b) const { return a * b; } };
+struct GzAdd { __device__ double operator()(double a, double b) const { return a + b; } };
+struct GzMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
+
 // Build the CDF window of one record with one wave, as 64-bit fixed point:
 // hi[0, n) the high words, hi[n, 2n) the low words.  Each lane takes a
 // contiguous segment; p(klo) = p0 comes from the prep kernel, and the pmf
@@ -467,12 +500,7 @@ __device__ __noinline__ void gz_build(uint32_t* hi, double lam, int klo, int n, 
     const double kb = (double)(klo + j0);
     double prod = 1.0;
     for (int j = 0; j < m; j++) prod *= lam * gz_rcp(kb + (double)(j + 1));
-    double px = prod;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double y = __shfl_up(px, off, 64);
-        if (lane >= off) px *= y;
-    }
+    const double px = gz_wave_scan(prod, lane, GzMul());
     double pre = __shfl_up(px, 1, 64);   // product over the lanes before this one
     if (lane == 0) pre = 1.0;
     const double pstart = p0 * pre;      // p(klo + j0)
@@ -484,24 +512,14 @@ __device__ __noinline__ void gz_build(uint32_t* hi, double lam, int klo, int n, 
             p *= lam * gz_rcp(kb + (double)(j + 1));
         }
     }
-    double x = local;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
+    const double x = gz_wave_scan(local, lane, GzAdd());
     const double carry = (lane > 0) ? x - local : 0.0;
     const double last = (m > 0) ? local + carry : -1.0;
     // The carries are rounded independently of the segment sums, so two
     // neighbouring segments can meet out of order by an ulp; raise each
     // segment to the largest value before it, which makes the table exactly
     // non-decreasing (so the inversion is a monotone map of the uniform).
-    double mx = last;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double y = __shfl_up(mx, off, 64);
-        if (lane >= off) mx = fmax(mx, y);
-    }
+    const double mx = gz_wave_scan(last, lane, GzMax());
     double before = __shfl_up(mx, 1, 64);
     if (lane == 0) before = 0.0;
     double p = pstart, acc = 0.0;
