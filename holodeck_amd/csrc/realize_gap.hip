@@ -50,7 +50,7 @@
 //                 without a table: reloads the partials, draws those records,
 //                 writes the partials back.
 // Per-(frequency, chunk, realization) partials (FP64 totals and the L best
-// candidates) are merged by realize.hip's merge kernel.
+// candidates) are merged by realize.hip's two merge kernels (chunk groups, then all).
 #include "common.hpp"
 #include "rng.hpp"
 #include "../../include/holohip.h"
