@@ -597,8 +597,16 @@ __device__ __forceinline__ void gz_table_draw32x2(const uint32_t* ha, int na, co
     gz_search_start(a, ha, na, xa);
     gz_search_start(b, hb, nb, xb);
     while (a.len > 1 && b.len > 1) {
-        gz_search_step(a);
-        gz_search_step(b);
+        // both loads issued before either compare, so their LDS latencies overlap
+        const int ha = a.len >> 1, hb = b.len >> 1;
+        const uint32_t* qa = a.p + ha;
+        const uint32_t* qb = b.p + hb;
+        const uint32_t va = *qa;
+        const uint32_t vb = *qb;
+        a.p = (va < a.x) ? qa : a.p;
+        b.p = (vb < b.x) ? qb : b.p;
+        a.len -= ha;
+        b.len -= hb;
     }
     while (a.len > 1) gz_search_step(a);
     while (b.len > 1) gz_search_step(b);
