@@ -33,6 +33,7 @@ SIGNATURES = {
     "hd_mass_stellar": (_I32, [_P, _I32, _P, _I32, _I32, _P, _P, _P, _P, _P]),
     "hd_find_2pwl_norm": (_I32, [_D, _P, _P, _I64, _D, _D, _D, _D, _I32, _P, _P, _P]),
     "hd_hard_func_2pwl_gw": (_I32, [_P, _P, _P, _P, _I64, _D, _D, _D, _P, _P]),
+    "hd_lifetime_2pwl": (_I32, [_P, _P, _P, _I64, _D, _D, _D, _D, _I32, _P, _P]),
     "hd_dbn_gw": (_I32, [_P, _I32, _P, _P, _I32, _P, _I32, _I32, _P, _P, _P, _I32, _P, _P, _P]),
     "hd_dbn_2pwl": (_I32, [_P, _I32, _I32, _D, _I32, _P, _D, _D, _D, _P, _P, _I32, _P, _I32, _P, _I32, _P,
                            _P, _P, _P, _I32, _P, _P, _P]),
@@ -52,9 +53,12 @@ SIGNATURES = {
     "hd_realize_workspace_size": (_SZ, [_I64, _I32, _I32, _I32, _I32]),
     "hd_realize": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _D, C.c_uint64, _P, _I32, _P, _P, _P,
                           _P, _P, _SZ, _P]),
+    "hd_realize_record_counts": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _D, C.c_uint64, _P, _I32,
+                                        _P, _P, _P, _P, _P, _P, _SZ, _P]),
     "hd_philox4x32_10": (None, [_P, _P, _P]),
     "hd_xoshiro128ss": (None, [_P, _I64, _P]),
     "hd_poisson_cdf_table": (_I32, [C.c_double, _I32, _P, _P, _P]),
+    "hd_poisson_window": (_I32, [C.c_double, _I32, _P, _P]),
     "hd_table_draw_words": (_I32, [_P, _I32, _P, _I64, _P, _P, _P]),
     "hd_table_draw_pair_words": (_I32, [_P, _I32, _P, _I32, _P, _I64, _P, _P, _P]),
     "hd_mlp_gemm_bf16": (_I32, [_P, _I64, _I64, _P, _I64, _I64, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _I64, _P]),
@@ -145,6 +149,14 @@ def timing_read(name):
     n = C.c_int(0)
     call("hd_timing_read", name.encode(), C.addressof(ms), C.addressof(n))
     return ms.value, n.value
+
+
+def poisson_window(lam, cap):
+    """(klo, n) of the realization kernels' CDF window for Poisson(lam) with cap table entries (host)."""
+    klo = C.c_int(0)
+    n = C.c_int(0)
+    call("hd_poisson_window", float(lam), int(cap), C.addressof(klo), C.addressof(n))
+    return klo.value, n.value
 
 
 def xoshiro128ss(state, n):

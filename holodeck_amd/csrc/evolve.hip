@@ -139,6 +139,34 @@ __global__ void __launch_bounds__(256) k_find_2pwl_norm(EvoConsts K, double targ
     }
 }
 
+// Lifetime from sepa_init to the ISCO at a given normalisation (no root finding), one wave per
+// element: integrate_binary_evolution_2pwl (sam_cyutils.pyx:401-418).
+__global__ void __launch_bounds__(256) k_lifetime_2pwl(EvoConsts K, const double* __restrict__ norm_log10,
+                                                       const double* __restrict__ mtot, const double* __restrict__ mrat,
+                                                       int64_t n, double sepa_init, double rchar, double g_in,
+                                                       double g_out, int nsteps, double* __restrict__ time) {
+    const int lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (item >= n) return;   // uniform per wave
+    LifeArgs a;
+    a.mt = mtot[item];
+    a.mr = mrat[item];
+    a.target = 0.0;
+    a.rchar = rchar;
+    a.g_in = g_in;
+    a.g_out = g_out;
+    const double x0 = log10(sepa_init);
+    a.dx = (x0 - log10(3.0 * MY_SCHW * a.mt)) / nsteps;
+    const int chunk = (nsteps + 63) / 64;
+    a.s_begin = min(lane * chunk, nsteps);
+    a.s_end = min(a.s_begin + chunk, nsteps);
+    double x = x0;
+    for (int s = 0; s < a.s_begin; s++) x -= a.dx;
+    a.x_lane0 = x;
+    const double t = lifetime_wave(K, a, norm_log10[item]);
+    if (lane == 0) time[item] = t;
+}
+
 __global__ void k_hard_func_2pwl_gw(EvoConsts K, const double* mtot, const double* mrat, const double* sepa,
                                     const double* norm, int64_t n, double rchar, double g_in, double g_out,
                                     double* dadt) {
@@ -483,6 +511,17 @@ extern "C" int hd_dbn_2pwl(const double* fobs_orb, int F, int fobs_sorted, doubl
                        fobs_sorted, sepa_init, num_steps, hard_norm, rchar, gamma_inner, gamma_outer, nden, mtot, M,
                        mrat, Q, redz, Z, gmt_time, grid_z, grid_dcom, grid_age, n_interp, redz_final, diff_num);
     return check_launch("k_dbn_2pwl");
+}
+
+extern "C" int hd_lifetime_2pwl(const double* norm_log10, const double* mtot, const double* mrat, int64_t n,
+                                double sepa_init, double rchar, double gamma_inner, double gamma_outer, int nsteps,
+                                double* time, void* stream) {
+    if (n < 0 || nsteps < 1 || (n > 0 && (!norm_log10 || !mtot || !mrat || !time)))
+        return set_error(HD_ERR_ARG, "hd_lifetime_2pwl: bad arguments");
+    if (n == 0) return HD_OK;
+    hipLaunchKernelGGL(k_lifetime_2pwl, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, (hipStream_t)stream, evo_consts(),
+                       norm_log10, mtot, mrat, n, sepa_init, rchar, gamma_inner, gamma_outer, nsteps, time);
+    return check_launch("k_lifetime_2pwl");
 }
 
 extern "C" int hd_integrate_3dx1d(const double* log10_mtot, int M, const double* mrat, int Q, const double* redz, int Z,

@@ -289,6 +289,12 @@ __global__ void k_rank_from_order(const int64_t* __restrict__ order, int64_t B, 
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+size_t realize_group_ws(int F, int R, int L, int npar);
+int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int F, int R, int L, int mode, int npar,
+                         int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
+                         const int32_t* p_bin, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
+                         void* group_ws, hipStream_t st);
+
 static size_t realize_ws(int64_t B, int F, int R, int L, int npar) {
     int64_t nch = cdiv(B, RZ_CHUNK);
     size_t pr = (size_t)F * nch * R;
@@ -298,6 +304,7 @@ static size_t realize_ws(int64_t B, int F, int R, int L, int npar) {
     s += align256(pr * 8 * (size_t)npar);       // param totals
     s += align256(pr * 8 * (size_t)L);          // keys
     s += align256(pr * 4 * (size_t)L);          // bins
+    s += align256(realize_group_ws(F, R, L, npar));   // first merge stage
     return s + 256;
 }
 
@@ -338,7 +345,8 @@ static int realize_counts_impl(const double* counts, const double* h2fdf, const 
     A.p_total = (double*)p;       p += align256(pr * 8);
     A.p_ptot = (double*)p;        p += align256(pr * 8 * (size_t)npar);
     A.p_key = (uint64_t*)p;       p += align256(pr * 8 * (size_t)L);
-    A.p_bin = (int32_t*)p;
+    A.p_bin = (int32_t*)p;        p += align256(pr * 4 * (size_t)L);
+    void* group_ws = p;
     A.rank = (mode == HD_SEL_SORTED) ? rank : nullptr;
 
     if (mode == HD_SEL_SORTED) {
@@ -351,9 +359,9 @@ static int realize_counts_impl(const double* counts, const double* h2fdf, const 
     hipLaunchKernelGGL(k_realize_chunk, grid, dim3(threads), 0, st, A);
     int rc = check_launch("k_realize_chunk");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_realize_merge, dim3((unsigned)cdiv((int64_t)F * R, 256)), dim3(256), 0, st, A, hc2ss, ssidx,
-                       hc2bg, (npar > 0) ? bgpar : nullptr);
-    return check_launch("k_realize_merge");
+    // the same two-stage merge as the drawn-count kernels (groups of chunks first when there are many)
+    return realize_merge_launch(h2fdf, par, B, F, R, L, mode, npar, A.nchunks, A.p_total, A.p_ptot, A.p_key, A.p_bin,
+                                hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, group_ws, st);
 }
 
 // Merge of per-chunk partials, shared with the drawn-count kernel (realize_gap.hip).

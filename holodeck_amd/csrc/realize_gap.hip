@@ -110,6 +110,7 @@ struct GapArgs {
     double* p_ptot;         // (npar, F, nchunks, R)
     uint64_t* p_key;        // (F, nchunks, L, R): realization fastest
     int32_t* p_bin;         // (F, nchunks, L, R)
+    double* counts_out;     // (B, F, R) every drawn count (test hook only; null in production)
 };
 
 __device__ __forceinline__ uint64_t gz_hkey(double h) {
@@ -118,13 +119,15 @@ __device__ __forceinline__ uint64_t gz_hkey(double h) {
     return ~u;
 }
 
-// CDF window of Poisson(lam): [klo, klo + n); n = 0 if it exceeds cap entries
-__device__ __forceinline__ void gz_window(double lam, int cap, int& klo, int& n) {
+// CDF window of Poisson(lam): [klo, klo + n); n = 0 (no table: the fallback sampler) if it
+// exceeds cap entries, or if its last count would not fit an int (lam near 2^31: the window
+// would then collapse to a single node by rounding, and klo overflow).
+__host__ __device__ __forceinline__ void gz_window(double lam, int cap, int& klo, int& n) {
     const double b = GZ_TAILB;
     const double lo = fmax(floor(lam - sqrt(2.0 * b * lam)), 0.0);
     const double hi = ceil(lam + b / 3.0 + sqrt(b * b / 9.0 + 2.0 * b * lam));
     const double w = hi - lo + 1.0;
-    if (!(w <= (double)cap)) {
+    if (!(w <= (double)cap) || !(hi < 2147483647.0)) {
         klo = 0;
         n = 0;
     } else {
@@ -288,6 +291,18 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
         A.cnt[ft * 3 + 1] = ns;
         A.cnt[ft * 3 + 2] = nbig;
     }
+}
+
+// Recording hook: the count drawn for (bin of `word`, f, r).  A bin is drawn by one thread per
+// (f, r), so plain stores suffice; sparse points add one each.
+__device__ __forceinline__ void gz_record(const GapArgs& A, uint64_t word, int f, int r, double n, bool add) {
+    double* p = A.counts_out + ((int64_t)(word >> 32) * A.F + f) * A.R + r;
+    *p = add ? *p + n : n;
+}
+
+__global__ void k_gz_zero(double* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -643,7 +658,9 @@ __device__ __forceinline__ void gz_store(const GapArgs& A, const Acc<MODE, LM, N
     }
 }
 
-template <int MODE, int LM, int NP>
+// REC: also write every drawn count to A.counts_out (the recording test hook); the production
+// instantiations have REC = false and carry no trace of it.
+template <int MODE, int LM, int NP, bool REC>
 // (6 waves per SIMD: with the LDS below three 512-thread workgroups share a CU)
 __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
     extern __shared__ double gz_lds[];
@@ -732,6 +749,10 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
                         if (na > 0) acc.add((double)(ra.klo + ka), ra.h, ra.word, A.d_par + reg + i, pstride, A.npar, L);
                         if (nb > 0)
                             acc.add((double)(rb.klo + kb), rb.h, rb.word, A.d_par + reg + i + 1, pstride, A.npar, L);
+                        if (REC) {
+                            if (na > 0) gz_record(A, ra.word, f, r, (double)(ra.klo + ka), false);
+                            if (nb > 0) gz_record(A, rb.word, f, r, (double)(rb.klo + kb), false);
+                        }
                     }
                     for (int i = max(c, nd); i < c1; i++) {
                         const BRec rec = brec[i - c];
@@ -752,6 +773,7 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
                                 }
                                 acc.add(1.0, A.s_h[reg + lo], A.s_word[reg + lo], A.sp_par + reg + lo, pstride, A.npar,
                                         L);
+                                if (REC) gz_record(A, A.s_word[reg + lo], f, r, 1.0, true);
                             }
                         }
                     }
@@ -765,7 +787,7 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
 
 // Records without a table (lambda above the threshold or a window larger than
 // the LDS table): reload the chunk's partials, draw, store them back.
-template <int MODE, int LM, int NP>
+template <int MODE, int LM, int NP, bool REC>
 __global__ void __launch_bounds__(256) k_gz_big(GapArgs A) {
     const int f = blockIdx.y;
     const int chunk = blockIdx.x;
@@ -796,6 +818,7 @@ __global__ void __launch_bounds__(256) k_gz_big(GapArgs A) {
                 const double n = gz_fallback((uint32_t)(rec.word >> 32), (uint32_t)f, (uint32_t)r, 3u, A.seed, rec.lam,
                                              A.thresh);
                 acc.add(n, rec.h, rec.word, A.d_par + ft * GZ_TILE + i, pstride, A.npar, L);
+                if (REC) gz_record(A, rec.word, f, r, n, false);
             }
         }
         gz_store<MODE, LM, NP>(A, acc, pr);
@@ -865,6 +888,13 @@ static int gz_block(int R) { return (int)min((int64_t)GZ_MAXBS, (int64_t)cdiv(R,
 #define HD_GZ_CAP 5632
 #endif
 static int gz_cap(int) { return HD_GZ_CAP; }
+// draw workgroup LDS: table + sparse axis + staged headers (+ the build counter); three must fit a
+// 160 KB CU (with the 64 KB per-workgroup limit of the allocation), and the hook's table as well
+constexpr size_t GZ_DRAW_LDS = (size_t)(HD_GZ_CAP + GZ_TILE) * sizeof(double) + (size_t)GZ_BATCH_REC * sizeof(BRec);
+static_assert(GZ_DRAW_LDS + 16 <= 65536, "draw workgroup LDS above 64 KB");
+static_assert(3 * (GZ_DRAW_LDS + 512) <= 160 * 1024, "three draw workgroups no longer share a CU");
+constexpr int GZ_HOOK_CAP_MAX = 8000;   // hd_poisson_cdf_table: 8-byte slots, 64000 B of LDS
+static_assert(GZ_HOOK_CAP_MAX * sizeof(double) <= 65536, "hook table above 64 KB");
 
 static size_t gz_ws(int64_t B, int F, int R, int L, int npar) {
     const int64_t nt = cdiv(B, GZ_TILE);
@@ -884,16 +914,22 @@ static size_t gz_ws(int64_t B, int F, int R, int L, int npar) {
     return s + 512;
 }
 
-template <int MODE, int LM, int NP>
-static int gz_launch(const GapArgs& A, dim3 grid, int bs, size_t lds, hipStream_t st) {
+template <int MODE, int LM, int NP, bool REC>
+static int gz_launch_k(const GapArgs& A, dim3 grid, int bs, size_t lds, hipStream_t st) {
     {
         KTimer timer("k_gz_draw", st);
-        hipLaunchKernelGGL((k_gz_draw<MODE, LM, NP>), grid, dim3(bs), lds, st, A);
+        hipLaunchKernelGGL((k_gz_draw<MODE, LM, NP, REC>), grid, dim3(bs), lds, st, A);
     }
     int rc = check_launch("k_gz_draw");
     if (rc) return rc;
-    hipLaunchKernelGGL((k_gz_big<MODE, LM, NP>), grid, dim3(256), 0, st, A);
+    hipLaunchKernelGGL((k_gz_big<MODE, LM, NP, REC>), grid, dim3(256), 0, st, A);
     return check_launch("k_gz_big");
+}
+
+template <int MODE, int LM, int NP>
+static int gz_launch(const GapArgs& A, dim3 grid, int bs, size_t lds, hipStream_t st) {
+    if (A.counts_out) return gz_launch_k<MODE, LM, NP, true>(A, grid, bs, lds, st);
+    return gz_launch_k<MODE, LM, NP, false>(A, grid, bs, lds, st);
 }
 
 template <int MODE, int LM>
@@ -911,10 +947,10 @@ extern "C" size_t hd_realize_workspace_size(int64_t B, int F, int R, int L, int 
     return gz_ws(B, F, R, L, npar);
 }
 
-extern "C" int hd_realize(const double* number, const double* h2fdf, const int64_t* order, int64_t B, int F, int R, int L,
-                          int mode, int flags, double normal_threshold, uint64_t seed, const double* par, int npar,
-                          double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar, void* workspace,
-                          size_t ws_bytes, void* stream) {
+static int realize_impl(const double* number, const double* h2fdf, const int64_t* order, int64_t B, int F, int R, int L,
+                        int mode, int flags, double normal_threshold, uint64_t seed, const double* par, int npar,
+                        double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar, double* counts_out,
+                        void* workspace, size_t ws_bytes, void* stream) {
     if (B < 1 || F < 1 || R < 1 || !number || !h2fdf || !hc2bg) return set_error(HD_ERR_ARG, "hd_realize: bad arguments");
     if (B > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: B=%lld exceeds 2^31-1", (long long)B);
     if (F > 65535) return set_error(HD_ERR_SIZE, "hd_realize: F=%d > 65535", F);
@@ -949,6 +985,7 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
     A.thresh = normal_threshold;
     A.tau = fmin(HD_GZ_TAU, normal_threshold);   // bins above the threshold are never sparse
     A.seed = seed;
+    A.counts_out = counts_out;
     A.ntiles = (int)cdiv(B, GZ_TILE);
     A.nchunks = (int)cdiv(A.ntiles, GZ_TILES_PER_CHUNK);
     const size_t slots = (size_t)F * A.ntiles * GZ_TILE;
@@ -974,13 +1011,18 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
         int rc = rank_from_order_launch(order, B, rank, st);
         if (rc) return rc;
     }
+    if (counts_out) {
+        hipLaunchKernelGGL(k_gz_zero, dim3(2048), dim3(256), 0, st, counts_out, B * (int64_t)F * R);
+        int rc = check_launch("k_gz_zero");
+        if (rc) return rc;
+    }
     const int64_t prep_blocks = cdiv(A.ntiles, GZ_XCDS) * GZ_XCDS * (int64_t)F;
     if (prep_blocks > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: too many tiles x frequencies");
     hipLaunchKernelGGL(k_gz_prep, dim3((unsigned)prep_blocks), dim3(GZ_TILE), 0, st, A);
     int rc = check_launch("k_gz_prep");
     if (rc) return rc;
     const dim3 grid(A.nchunks, F);
-    const size_t lds = (size_t)(A.cap + GZ_TILE) * sizeof(double) + (size_t)GZ_BATCH_REC * sizeof(BRec);
+    const size_t lds = GZ_DRAW_LDS;
     if (mode == HD_SEL_SORTED) {
         if (L == 1) rc = gz_launch_np<HD_SEL_SORTED, 1>(A, npar, grid, bs, lds, st);
         else if (L == 5) rc = gz_launch_np<HD_SEL_SORTED, 5>(A, npar, grid, bs, lds, st);
@@ -999,9 +1041,38 @@ extern "C" int hd_realize(const double* number, const double* h2fdf, const int64
                                 hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, group_ws, st);
 }
 
+extern "C" int hd_realize(const double* number, const double* h2fdf, const int64_t* order, int64_t B, int F, int R, int L,
+                          int mode, int flags, double normal_threshold, uint64_t seed, const double* par, int npar,
+                          double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar, void* workspace,
+                          size_t ws_bytes, void* stream) {
+    return realize_impl(number, h2fdf, order, B, F, R, L, mode, flags, normal_threshold, seed, par, npar, hc2ss, ssidx,
+                        hc2bg, bgpar, nullptr, workspace, ws_bytes, stream);
+}
+
+extern "C" int hd_realize_record_counts(const double* number, const double* h2fdf, const int64_t* order, int64_t B,
+                                        int F, int R, int L, int mode, int flags, double normal_threshold,
+                                        uint64_t seed, const double* par, int npar, double* hc2ss, int64_t* ssidx,
+                                        double* hc2bg, double* bgpar, double* counts_out, void* workspace,
+                                        size_t ws_bytes, void* stream) {
+    if (!counts_out) return set_error(HD_ERR_ARG, "hd_realize_record_counts: counts_out required");
+    return realize_impl(number, h2fdf, order, B, F, R, L, mode, flags, normal_threshold, seed, par, npar, hc2ss, ssidx,
+                        hc2bg, bgpar, counts_out, workspace, ws_bytes, stream);
+}
+
+extern "C" int hd_poisson_window(double lam, int cap, int* klo, int* n) {
+    if (!(lam > 0.0) || !(lam < 1e300) || cap < 1 || !klo || !n) return set_error(HD_ERR_ARG, "hd_poisson_window: bad arguments");
+    gz_window(lam, cap, *klo, *n);
+    return HD_OK;
+}
+
 extern "C" int hd_poisson_cdf_table(double lam, int cap, int* klo_n, uint64_t* cdf, void* stream) {
-    if (!(lam > 0.0) || !(lam < 1e300) || cap < 1 || cap > 8000 || !klo_n || !cdf)
+    if (!(lam > 0.0) || !(lam < 1e300) || cap < 1 || cap > GZ_HOOK_CAP_MAX || !klo_n || !cdf)
         return set_error(HD_ERR_ARG, "hd_poisson_cdf_table: bad arguments");
+    {   // the window is checked on the host first: a table only when it fits
+        int klo, n;
+        gz_window(lam, cap, klo, n);
+        if (n < 1) return set_error(HD_ERR_SIZE, "hd_poisson_cdf_table: window of lam=%g exceeds cap=%d", lam, cap);
+    }
     hipLaunchKernelGGL(k_cdf_table_hook, dim3(1), dim3(64), (size_t)cap * sizeof(double), (hipStream_t)stream, lam,
                        cap, klo_n, cdf);
     return check_launch("k_cdf_table_hook");

@@ -30,8 +30,15 @@ def _counts(counts, B, F):
     return None if counts is None else D.to_dev(np.asarray(counts, dtype=float)).reshape(B, F, -1).contiguous()
 
 
+def _long_threshold(t):
+    """The reference's Cython loops take the threshold as ``long thresh`` (truncated toward zero)."""
+    t = float(t)
+    return float(int(t)) if np.isfinite(t) else t
+
+
 def _run(number, h2fdf, nreals, nloudest, mode, order=None, flags=0, par=None, normal_threshold=1e10, seed=None,
          counts=None):
+    normal_threshold = _long_threshold(normal_threshold)
     n, h, shape = _prep(number, h2fdf)
     B, F = h.shape
     cnt = _counts(counts, B, F)
@@ -53,6 +60,40 @@ def loudest_hc_from_sorted(number, h2fdf, nreals, nloudest, msort, qsort, zsort,
     (hc2ss, _, hc2bg, _), _ = _run(number, h2fdf, nreals, nloudest, _lib.HD_SEL_SORTED, order=order,
                                    normal_threshold=normal_threshold, seed=seed, counts=counts)
     return D.to_host(hc2ss), D.to_host(hc2bg)
+
+
+def loudest_hc_and_par_from_sorted(number, h2fdf, nreals, nloudest, mt, mr, rz, msort, qsort, zsort,
+                                   normal_threshold=1e10, *, seed=None, counts=None):
+    """-> (hc2ss (F,R,L), hc2bg (F,R), lspar (3,F,R), bgpar (3,F,R), ssidx (3,F,R,L))
+    (cyutils.pyx:1347-1538).
+
+    The frequency-0 order walk with the strain-weighted mean (M, q, z) of the loud instances
+    (lspar; 0/0 = NaN where there is none, as the reference's C division) and of the background;
+    zero-strain bins are not skipped (unlike the _redz variant).  ssidx holds the (M, Q, Z) bin
+    indices of each loud slot, 0 where a slot is empty (the reference's np.zeros).  The threshold
+    is taken as an integer, as the reference's ``long thresh`` argument."""
+    number = np.asarray(number, dtype=float)
+    shape3 = number.shape[:3]
+    F = number.shape[-1]
+    mt, mr, rz = (np.asarray(a, dtype=float) for a in (mt, mr, rz))
+    mm, qq, zz = np.meshgrid(np.arange(shape3[0]), np.arange(shape3[1]), np.arange(shape3[2]), indexing='ij')
+    pars = [mt[mm], mr[qq], rz[zz]]
+    par = D.to_dev(np.stack([np.repeat(p.reshape(-1, 1), F, axis=1) for p in pars]))
+    order = _order_from_sorts(shape3, msort, qsort, zsort)
+    (hc2ss, ssidx, hc2bg, bgpar), _ = _run(number, h2fdf, nreals, nloudest, _lib.HD_SEL_SORTED, order=order,
+                                           par=par, normal_threshold=normal_threshold, seed=seed, counts=counts)
+    ss = D.to_host(hc2ss)
+    idx = D.to_host(ssidx)
+    ok = idx >= 0
+    m3 = np.unravel_index(np.where(ok, idx, 0), shape3)
+    out_idx = np.zeros((3,) + idx.shape, dtype=np.int64)
+    for k in range(3):
+        out_idx[k][ok] = m3[k][ok]
+    vals = (mt[m3[0]], mr[m3[1]], rz[m3[2]])
+    sum_ls = ss.sum(axis=-1)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        lspar = np.stack([np.where(ok, ss * v, 0.0).sum(axis=-1) / sum_ls for v in vals])
+    return ss, D.to_host(hc2bg), lspar, D.to_host(bgpar), out_idx
 
 
 def loudest_hc_and_par_from_sorted_redz(number, h2fdf, nreals, nloudest, mt, mr, rz, redz_final, dcom_final, sepa,

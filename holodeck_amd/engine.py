@@ -136,8 +136,11 @@ def order_desc(key2d, col=0):
 
 
 def realize(number2d, h2fdf2d, nreals, nloud, mode, order=None, counts=None, flags=0, par=None, seed=0,
-            normal_threshold=NORMAL_THRESHOLD):
-    """Realizations over (B,F) bins.  Returns (hc2ss (F,R,L), ssidx (F,R,L), hc2bg (F,R), bgpar (npar,F,R)|None)."""
+            normal_threshold=NORMAL_THRESHOLD, record_counts=None):
+    """Realizations over (B,F) bins.  Returns (hc2ss (F,R,L), ssidx (F,R,L), hc2bg (F,R), bgpar (npar,F,R)|None).
+
+    ``record_counts``: a (B, F, R) float64 device tensor that receives every count the draw kernels
+    draw (the recording test hook; same outputs as without it for the same seed)."""
     B, F = h2fdf2d.shape
     R = int(nreals)
     L = 0 if mode == _lib.HD_SEL_NONE else int(nloud)
@@ -151,6 +154,14 @@ def realize(number2d, h2fdf2d, nreals, nloud, mode, order=None, counts=None, fla
             ws = D.workspace(_lib.size("hd_realize_counts_workspace_size", B, F, R, L, npar))
             _lib.call("hd_realize_counts", D.ptr(counts), D.ptr(h2fdf2d), D.ptr(order), B, F, R, L, mode, flags,
                       D.ptr(par), npar, D.ptr(hc2ss), D.ptr(ssidx), D.ptr(hc2bg), D.ptr(bgpar), D.ptr(ws), ws.numel(),
+                      D.stream_ptr())
+        elif record_counts is not None:
+            if tuple(record_counts.shape) != (B, F, R) or record_counts.dtype != torch.float64:
+                raise ValueError(f"record_counts must be float64 of shape {(B, F, R)}")
+            ws = D.workspace(_lib.size("hd_realize_workspace_size", B, F, R, L, npar))
+            _lib.call("hd_realize_record_counts", D.ptr(number2d), D.ptr(h2fdf2d), D.ptr(order), B, F, R, L, mode,
+                      flags, float(normal_threshold), int(seed) & 0xFFFFFFFFFFFFFFFF, D.ptr(par), npar, D.ptr(hc2ss),
+                      D.ptr(ssidx), D.ptr(hc2bg), D.ptr(bgpar), D.ptr(record_counts), D.ptr(ws), ws.numel(),
                       D.stream_ptr())
         else:
             ws = D.workspace(_lib.size("hd_realize_workspace_size", B, F, R, L, npar))

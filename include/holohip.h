@@ -116,6 +116,10 @@ int hd_scatter_weights_T(const double *dm, int G, double *wT, void *stream);
 int hd_find_2pwl_norm(double target_time, const double *mtot, const double *mrat, int64_t n, double sepa_init,
                       double rchar, double gamma_inner, double gamma_outer, int nsteps, double *norm_log10,
                       double *norm, void *stream);
+/* integrate_binary_evolution_2pwl (sam_cyutils.pyx:401-418), elementwise over n: the 300-step
+ * (nsteps) lifetime [s] from sepa_init to 3 Schwarzschild radii at normalisation 10**norm_log10. */
+int hd_lifetime_2pwl(const double *norm_log10, const double *mtot, const double *mrat, int64_t n, double sepa_init,
+                     double rchar, double gamma_inner, double gamma_outer, int nsteps, double *time, void *stream);
 /* hard_func_2pwl_gw (sam_cyutils.pyx:271-286), elementwise over n. */
 int hd_hard_func_2pwl_gw(const double *mtot, const double *mrat, const double *sepa, const double *norm, int64_t n,
                          double rchar, double gamma_inner, double gamma_outer, double *dadt, void *stream);
@@ -213,6 +217,16 @@ int hd_realize(const double *number, const double *h2fdf, const int64_t *order, 
                double *hc2ss, int64_t *ssidx, double *hc2bg, double *bgpar, void *workspace, size_t ws_bytes,
                void *stream);
 
+/* Test hook of hd_realize (no counterpart in the reference, whose Cython loops draw each count
+ * and use it at once, cyutils.pyx:1266-1344, 1615-1767, 1068-1178, 862-897): the same launch
+ * sequence and outputs as hd_realize for the same seed, and in addition every count the draw
+ * kernels drew, counts_out (B,F,R) (0 for bins that are never drawn: number 0, or excluded by the
+ * mode's rules), so that the reference's walks can be run on exactly those counts. */
+int hd_realize_record_counts(const double *number, const double *h2fdf, const int64_t *order, int64_t B, int F,
+                             int R, int L, int mode, int flags, double normal_threshold, uint64_t seed,
+                             const double *par, int npar, double *hc2ss, int64_t *ssidx, double *hc2bg,
+                             double *bgpar, double *counts_out, void *workspace, size_t ws_bytes, void *stream);
+
 /* Philox4x32-10 block function (host), for known-answer tests of the
  * generator the realization kernels use. */
 void hd_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
@@ -226,6 +240,9 @@ void hd_xoshiro128ss(uint32_t *state, int64_t n, uint32_t *out);
  * inversions of such a table with given words (tab = the n high words then the n low words;
  * two words per draw; slot[i] the window index drawn, used[i] the words it took). */
 int hd_poisson_cdf_table(double lam, int cap, int *klo_n, uint64_t *cdf, void *stream);
+/* The same window on the host (no GPU work): [klo, klo + n) with each dropped tail below 2^-56;
+ * n = 0 when it needs more than cap entries or its end would not fit an int. */
+int hd_poisson_window(double lam, int cap, int *klo, int *n);
 int hd_table_draw_words(const uint32_t *tab, int n, const uint32_t *words, int64_t m, int *slot, int *used,
                         void *stream);
 /* The same for two tables drawn together (the kernels' interleaved pair): four words per pair,
