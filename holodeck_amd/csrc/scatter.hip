@@ -101,69 +101,201 @@ __global__ void __launch_bounds__(256) k_transpose_nz(const double* __restrict__
     b[z * N + p] = a[t];
 }
 
+// Fixed-width copy of the slot-ordered geometry: the first CT_D edges of every
+// slot in [d][slot] arrays (padding: neighbour 0, zero edge, L^3 = 1, never read
+// as the slot's degree stops the loop), so a sweep level loads each slot's edges
+// with independent, coalesced reads instead of chasing slot_eoff.
+constexpr int CT_D = 8;
+__global__ void __launch_bounds__(256) k_ct_pad(int N, const int32_t* __restrict__ slot_eoff,
+                                                const int32_t* __restrict__ e_q, const double* __restrict__ e_geo,
+                                                int32_t* __restrict__ s_deg, int32_t* __restrict__ pq,
+                                                double* __restrict__ pex, double* __restrict__ pey,
+                                                double* __restrict__ pl3) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int e0 = slot_eoff[t], deg = slot_eoff[t + 1] - e0;
+    s_deg[t] = deg;
+#pragma unroll
+    for (int d = 0; d < CT_D; d++) {
+        const bool in = d < deg;
+        pq[d * N + t] = in ? e_q[e0 + d] : 0;
+        pex[d * N + t] = in ? e_geo[3 * (e0 + d)] : 0.0;
+        pey[d * N + t] = in ? e_geo[3 * (e0 + d) + 1] : 0.0;
+        pl3[d * N + t] = in ? e_geo[3 * (e0 + d) + 2] : 1.0;
+    }
+}
+
+// The per-slice constant of every padded edge, 6 (f1 - f2) (the sweep's first
+// term, the same operation on the same values), A[z][d][slot].
+__global__ void __launch_bounds__(256) k_ct_fdiff(const double* __restrict__ densT, int N, int Z,
+                                                  const int32_t* __restrict__ lidx, const int32_t* __restrict__ s_deg,
+                                                  const int32_t* __restrict__ pq, double* __restrict__ A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)N * Z) return;
+    const int z = (int)(i / N), t = (int)(i % N);
+    const double* frow = densT + (int64_t)z * N;
+    const double f1 = frow[lidx[t]];
+    const int deg = s_deg[t];
+#pragma unroll
+    for (int d = 0; d < CT_D; d++)
+        A[((int64_t)z * CT_D + d) * N + t] = (d < deg) ? 6 * (f1 - frow[pq[d * N + t]]) : 0.0;
+}
+
+// One slot's inputs to a sweep step, held in registers (loaded one level ahead).
+struct CtSlot {
+    int p, deg, e0;
+    double Q0, Q1, Q3, det, f1;
+    int q[CT_D];
+    double ex[CT_D], ey[CT_D], l3[CT_D], a[CT_D];
+};
+
+__device__ __forceinline__ void ct_load(CtSlot& s, int t, bool valid, int N, const double* __restrict__ frow,
+                                        const double* __restrict__ Az, const int32_t* __restrict__ lidx,
+                                        const int32_t* __restrict__ s_deg, const int32_t* __restrict__ slot_eoff,
+                                        const double* __restrict__ slot_q, const int32_t* __restrict__ pq,
+                                        const double* __restrict__ pex, const double* __restrict__ pey,
+                                        const double* __restrict__ pl3) {
+    s.deg = 0;
+    if (!valid) return;
+    s.p = lidx[t];
+    s.deg = s_deg[t];
+    s.e0 = slot_eoff[t];
+    s.Q0 = slot_q[4 * t];
+    s.Q1 = slot_q[4 * t + 1];
+    s.Q3 = slot_q[4 * t + 2];
+    s.det = slot_q[4 * t + 3];
+    s.f1 = frow[s.p];
+#pragma unroll
+    for (int d = 0; d < CT_D; d++) {
+        s.q[d] = pq[d * N + t];
+        s.ex[d] = pex[d * N + t];
+        s.ey[d] = pey[d * N + t];
+        s.l3[d] = pl3[d * N + t];
+        s.a[d] = Az[(int64_t)d * N + t];
+    }
+}
+
+// One Gauss-Seidel update of slot s: scipy's arithmetic, term by term and in the
+// same order (the first CT_D edges from registers, the rest of a wider slot from
+// the slot-ordered CSR).  Returns the relative change.
+__device__ __forceinline__ double ct_step(const CtSlot& c, double* y, const double* __restrict__ frow,
+                                          const int32_t* __restrict__ e_q, const double* __restrict__ e_geo) {
+    double s0 = 0, s1 = 0;
+#pragma unroll
+    for (int d = 0; d < CT_D; d++) {
+        if (d < c.deg) {
+            const int q = c.q[d];
+            const double ex = c.ex[d], ey = c.ey[d], L3 = c.l3[d];
+            const double df2 = -ex * y[2 * q] - ey * y[2 * q + 1];
+            s0 += (c.a[d] - 2 * df2) * ex / L3;
+            s1 += (c.a[d] - 2 * df2) * ey / L3;
+        }
+    }
+    for (int e = c.e0 + CT_D; e < c.e0 + c.deg; e++) {
+        const int q = e_q[e];
+        const double ex = e_geo[3 * e], ey = e_geo[3 * e + 1], L3 = e_geo[3 * e + 2];
+        const double f2 = frow[q];
+        const double df2 = -ex * y[2 * q] - ey * y[2 * q + 1];
+        s0 += (6 * (c.f1 - f2) - 2 * df2) * ex / L3;
+        s1 += (6 * (c.f1 - f2) - 2 * df2) * ey / L3;
+    }
+    const double Q0 = c.Q0, Q1 = c.Q1, Q3 = c.Q3, det = c.det;
+    const double Q2 = Q1;
+    const double r0 = (Q3 * s0 - Q1 * s1) / det;
+    const double r1 = (-Q2 * s0 + Q0 * s1) / det;
+    const int p = c.p;
+    double a0 = fabs(y[2 * p] + r0), a1 = fabs(y[2 * p + 1] + r1);
+    double change = (a0 > a1) ? a0 : a1;
+    y[2 * p] = -r0;
+    y[2 * p + 1] = -r1;
+    double m0 = fabs(r0), m1 = fabs(r1);
+    double mr = (m0 > m1) ? m0 : m1;
+    change /= (1.0 > mr) ? 1.0 : mr;
+    return change;
+}
+
+// Gauss-Seidel sweeps of one slice (one wave).  Each level's slot inputs are
+// loaded while the previous level is computed (they do not depend on the
+// solution), into the other of two register buffers, so a level costs its LDS
+// reads of the neighbours' gradients and its arithmetic.  With the solution in
+// LDS the single wave's LDS operations complete in issue order, so a level's
+// reads see the previous level's writes without a barrier.
 template <bool LDS_Y>
 __global__ void __launch_bounds__(64) k_ct_gradients(const double* __restrict__ densT, int N, int Z,
                                                      const int32_t* __restrict__ lidx,
                                                      const int32_t* __restrict__ slot_eoff,
                                                      const double* __restrict__ slot_q, const int32_t* __restrict__ e_q,
-                                                     const double* __restrict__ e_geo, const int32_t* __restrict__ lptr,
+                                                     const double* __restrict__ e_geo, const int32_t* __restrict__ s_deg,
+                                                     const int32_t* __restrict__ pq, const double* __restrict__ pex,
+                                                     const double* __restrict__ pey, const double* __restrict__ pl3,
+                                                     const double* __restrict__ A, const int32_t* __restrict__ lptr,
                                                      int n_levels, double tol, int maxiter, double* __restrict__ grad_g,
                                                      int32_t* __restrict__ iters_out) {
     extern __shared__ double s_y[];
     const int z = blockIdx.x;
     const int lane = threadIdx.x;
     const double* frow = densT + (int64_t)z * N;
+    const double* Az = A + (int64_t)z * CT_D * N;
     double* y = LDS_Y ? s_y : (grad_g + (int64_t)z * N * 2);
+    // the level pointers next to the solution (or alone) in LDS
+    int32_t* s_lptr = (int32_t*)(LDS_Y ? (s_y + 2 * (size_t)N) : s_y);
     for (int i = lane; i < 2 * N; i += 64) y[i] = 0.0;
+    for (int i = lane; i <= n_levels; i += 64) s_lptr[i] = lptr[i];
     __syncthreads();
-    int iters = 0;
-    for (int it = 0; it < maxiter; it++) {
-        double err = 0.0;
-        for (int lev = 0; lev < n_levels; lev++) {
-            const int l0 = lptr[lev], l1 = lptr[lev + 1];
-            for (int t = l0 + lane; t < l1; t += 64) {
-                const int p = lidx[t];
-                const int e0 = slot_eoff[t], e1 = slot_eoff[t + 1];
-                const double f1 = frow[p];
-                const double Q0 = slot_q[4 * t], Q1 = slot_q[4 * t + 1], Q3 = slot_q[4 * t + 2];
-                const double det = slot_q[4 * t + 3];
-                double s0 = 0, s1 = 0;
-#pragma unroll 4
-                for (int e = e0; e < e1; e++) {
-                    const int q = e_q[e];
-                    const double ex = e_geo[3 * e], ey = e_geo[3 * e + 1], L3 = e_geo[3 * e + 2];
-                    const double f2 = frow[q];
-                    const double df2 = -ex * y[2 * q] - ey * y[2 * q + 1];
-                    s0 += (6 * (f1 - f2) - 2 * df2) * ex / L3;
-                    s1 += (6 * (f1 - f2) - 2 * df2) * ey / L3;
-                }
-                const double Q2 = Q1;
-                const double r0 = (Q3 * s0 - Q1 * s1) / det;
-                const double r1 = (-Q2 * s0 + Q0 * s1) / det;
-                double a0 = fabs(y[2 * p] + r0), a1 = fabs(y[2 * p + 1] + r1);
-                double change = (a0 > a1) ? a0 : a1;
-                y[2 * p] = -r0;
-                y[2 * p + 1] = -r1;
-                double m0 = fabs(r0), m1 = fabs(r1);
-                double mr = (m0 > m1) ? m0 : m1;
-                change /= (1.0 > mr) ? 1.0 : mr;
-                err = (err > change) ? err : change;
-            }
-            if (LDS_Y) __syncthreads();
-            else __threadfence_block();
+    auto load = [&](CtSlot& b, int lev) {
+        const int t = s_lptr[lev] + lane;
+        ct_load(b, t, t < s_lptr[lev + 1], N, frow, Az, lidx, s_deg, slot_eoff, slot_q, pq, pex, pey, pl3);
+    };
+    auto level_done = [&]() {
+        if (LDS_Y) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __threadfence_block();
         }
+    };
+    CtSlot b0, b1;
+    load(b0, 0);
+    int iters = 0, it = 0, lev = 0;
+    double err = 0.0;
+    // the steps (sweep, level) in order, two per trip: b0 holds the even steps, b1 the odd
+    // ones; returns true when a sweep ends converged or at maxiter
+    auto finish_level = [&]() -> bool {
+        level_done();
+        if (++lev < n_levels) return false;
+        lev = 0;
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             double o = __shfl_xor(err, off, 64);
             err = (err > o) ? err : o;
         }
+        it++;
         if (err < tol) {
-            iters = it + 1;
-            break;
+            iters = it;
+            return true;
         }
+        err = 0.0;
+        return it >= maxiter;
+    };
+    for (;;) {
+        load(b1, (lev + 1 < n_levels) ? lev + 1 : 0);
+        if (b0.deg > 0) {
+            const double ch = ct_step(b0, y, frow, e_q, e_geo);
+            err = (err > ch) ? err : ch;
+        }
+        if (finish_level()) break;
+        load(b0, (lev + 1 < n_levels) ? lev + 1 : 0);
+        if (b1.deg > 0) {
+            const double ch = ct_step(b1, y, frow, e_q, e_geo);
+            err = (err > ch) ? err : ch;
+        }
+        if (finish_level()) break;
     }
-    if (LDS_Y)
+    if (LDS_Y) {
+        __syncthreads();
         for (int i = lane; i < 2 * N; i += 64) grad_g[(int64_t)z * N * 2 + i] = y[i];
+    }
     if (lane == 0) iters_out[z] = iters;
 }
 
@@ -347,7 +479,9 @@ extern "C" size_t hd_scatter_workspace_size(int M, int Q, int Z, int G) {
     size_t N = (size_t)M * Q;
     const size_t ecap = 6 * N;
     return align256(N * Z * 2 * 8) + 2 * align256((size_t)G * G * Z * 8) + align256(N * Z * 8) +   // grad, D, Y, densT
-           align256((N + 1) * 4) + align256(N * 4 * 8) + align256(ecap * 4) + align256(ecap * 3 * 8) + 256;
+           align256((N + 1) * 4) + align256(N * 4 * 8) + align256(ecap * 4) + align256(ecap * 3 * 8) +
+           align256(N * 4) + align256(CT_D * N * 4) + 3 * align256(CT_D * N * 8) +   // padded geometry
+           align256((size_t)Z * CT_D * N * 8) + 256;                              // 6 (f1 - f2) per slice
 }
 
 extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double* pts, const int32_t* nbr_ptr,
@@ -374,7 +508,13 @@ extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double
     int32_t* slot_eoff = (int32_t*)p;  p += align256((size_t)(N + 1) * 4);
     double* slot_q = (double*)p;       p += align256((size_t)N * 4 * 8);
     int32_t* e_q = (int32_t*)p;        p += align256((size_t)ecap * 4);
-    double* e_geo = (double*)p;
+    double* e_geo = (double*)p;        p += align256((size_t)ecap * 3 * 8);
+    int32_t* s_deg = (int32_t*)p;      p += align256((size_t)N * 4);
+    int32_t* pq = (int32_t*)p;         p += align256((size_t)CT_D * N * 4);
+    double* pex = (double*)p;          p += align256((size_t)CT_D * N * 8);
+    double* pey = (double*)p;          p += align256((size_t)CT_D * N * 8);
+    double* pl3 = (double*)p;          p += align256((size_t)CT_D * N * 8);
+    double* fdiff = (double*)p;
     int32_t* bad = iters_out + Z;   // iters_out has Z + 1 entries
 
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, bad, 1);
@@ -382,23 +522,39 @@ extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double
     hipLaunchKernelGGL(k_ct_geom, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, st, N, pts, nbr_ptr, nbr_idx, levels_idx,
                        slot_eoff, ecap, slot_q, e_q, e_geo, bad);
     hipLaunchKernelGGL(k_transpose_nz, dim3((unsigned)cdiv((int64_t)N * Z, 256)), dim3(256), 0, st, dens, N, Z, densT);
-    int rc0 = check_launch("k_ct_geom");
+    hipLaunchKernelGGL(k_ct_pad, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, st, N, slot_eoff, e_q, e_geo, s_deg, pq,
+                       pex, pey, pl3);
+    hipLaunchKernelGGL(k_ct_fdiff, dim3((unsigned)cdiv((int64_t)N * Z, 256)), dim3(256), 0, st, densT, N, Z,
+                       levels_idx, s_deg, pq, fdiff);
+    int rc0 = check_launch("k_ct_fdiff");
     if (rc0) return rc0;
-    // solution vector in LDS when it fits
+    // solution vector (and the level pointers) in LDS when they fit
     int dev = 0;
-    hipGetDevice(&dev);
+    hipError_t he = hipGetDevice(&dev);
     int max_lds = 0;
-    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    size_t y_bytes = (size_t)N * 2 * sizeof(double);
+    if (he == hipSuccess) he = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (he != hipSuccess) return set_error(HD_ERR_HIP, "hd_scatter: device query: %s", hipGetErrorString(he));
+    const size_t lptr_bytes = (size_t)(n_levels + 1) * sizeof(int32_t);
+    const size_t y_bytes = (size_t)N * 2 * sizeof(double) + lptr_bytes;
+    if (lptr_bytes > (size_t)max_lds) return set_error(HD_ERR_SIZE, "hd_scatter: %d levels exceed the LDS", n_levels);
     if (y_bytes <= (size_t)max_lds) {
-        if (y_bytes > 65536)
-            hipFuncSetAttribute((const void*)k_ct_gradients<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)y_bytes);
+        if (y_bytes > 65536) {
+            he = hipFuncSetAttribute((const void*)k_ct_gradients<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)y_bytes);
+            if (he != hipSuccess) return set_error(HD_ERR_HIP, "hd_scatter: LDS attribute: %s", hipGetErrorString(he));
+        }
         hipLaunchKernelGGL(k_ct_gradients<true>, dim3(Z), dim3(64), y_bytes, st, densT, N, Z, levels_idx, slot_eoff,
-                           slot_q, e_q, e_geo, levels_ptr, n_levels, tol, maxiter, grad, iters_out);
+                           slot_q, e_q, e_geo, s_deg, pq, pex, pey, pl3, fdiff, levels_ptr, n_levels, tol, maxiter,
+                           grad, iters_out);
     } else {
-        hipLaunchKernelGGL(k_ct_gradients<false>, dim3(Z), dim3(64), 0, st, densT, N, Z, levels_idx, slot_eoff, slot_q,
-                           e_q, e_geo, levels_ptr, n_levels, tol, maxiter, grad, iters_out);
+        if (lptr_bytes > 65536) {
+            he = hipFuncSetAttribute((const void*)k_ct_gradients<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lptr_bytes);
+            if (he != hipSuccess) return set_error(HD_ERR_HIP, "hd_scatter: LDS attribute: %s", hipGetErrorString(he));
+        }
+        hipLaunchKernelGGL(k_ct_gradients<false>, dim3(Z), dim3(64), lptr_bytes, st, densT, N, Z, levels_idx, slot_eoff,
+                           slot_q, e_q, e_geo, s_deg, pq, pex, pey, pl3, fdiff, levels_ptr, n_levels, tol, maxiter,
+                           grad, iters_out);
     }
     int rc = check_launch("k_ct_gradients");
     if (rc) return rc;
