@@ -55,6 +55,8 @@ SIGNATURES = {
                           _P, _P, _SZ, _P]),
     "hd_realize_record_counts": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _D, C.c_uint64, _P, _I32,
                                         _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "hd_realize_noalloc": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _D, C.c_uint64, _P, _I32, _P, _P,
+                                  _P, _P, _P]),
     "hd_philox4x32_10": (None, [_P, _P, _P]),
     "hd_xoshiro128ss": (None, [_P, _I64, _P]),
     "hd_poisson_cdf_table": (_I32, [C.c_double, _I32, _P, _P, _P]),

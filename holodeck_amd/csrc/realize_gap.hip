@@ -54,6 +54,7 @@
 #include "common.hpp"
 #include "rng.hpp"
 #include "../../include/holohip.h"
+#include <mutex>
 
 namespace hd {
 
@@ -1057,6 +1058,77 @@ extern "C" int hd_realize_record_counts(const double* number, const double* h2fd
     if (!counts_out) return set_error(HD_ERR_ARG, "hd_realize_record_counts: counts_out required");
     return realize_impl(number, h2fdf, order, B, F, R, L, mode, flags, normal_threshold, seed, par, npar, hc2ss, ssidx,
                         hc2bg, bgpar, counts_out, workspace, ws_bytes, stream);
+}
+
+// ---------------------------------------------------------------------------
+// hd_realize_noalloc: hd_realize's launch sequence on a fixed device array
+// ---------------------------------------------------------------------------
+// The workspace of hd_realize (records, sparse axes, partials, merge stage) lives
+// in this file-scope array instead of a caller allocation.  Every GPU the library
+// runs on holds its own copy, HD_NOALLOC_SCRATCH_BYTES, until the process ends.
+// Host code never passes `scratch` itself to a kernel (its host-side name is a
+// stand-in, not a device address): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) unsigned char scratch[HD_NOALLOC_SCRATCH_BYTES];
+
+namespace hd {
+constexpr int GZ_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[GZ_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up on the first call per device).
+static int scratch_address(hipStream_t st, void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(HD_ERR_HIP, "hd_realize_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= GZ_MAX_DEVICES)
+        return set_error(HD_ERR_ARG, "hd_realize_noalloc: device %d beyond %d", dev, GZ_MAX_DEVICES);
+    if (st) {   // the scratch of the stream's device is the one the kernels will touch
+        hipDevice_t sdev = -1;
+        e = hipStreamGetDevice(st, &sdev);
+        if (e != hipSuccess)
+            return set_error(HD_ERR_HIP, "hd_realize_noalloc: hipStreamGetDevice: %s", hipGetErrorString(e));
+        if (sdev != dev)
+            return set_error(HD_ERR_ARG, "hd_realize_noalloc: stream of device %d, current device %d", (int)sdev, dev);
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(HD_ERR_HIP, "hd_realize_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return HD_OK;
+}
+}  // namespace hd
+
+extern "C" int hd_realize_noalloc(const double* number, const double* h2fdf, const int64_t* order, int64_t B, int F,
+                                  int R, int L, int mode, int flags, double normal_threshold, uint64_t seed,
+                                  const double* par, int npar, double* hc2ss, int64_t* ssidx, double* hc2bg,
+                                  double* bgpar, void* stream) {
+    // the sizes are checked before the scratch is sized against them (gz_ws needs sane values)
+    if (B < 1 || F < 1 || R < 1) return set_error(HD_ERR_ARG, "hd_realize_noalloc: bad arguments");
+    if (B > 0x7fffffffll || F > 65535 || R > 0x7fffffff / 2)
+        return set_error(HD_ERR_SIZE, "hd_realize_noalloc: sizes B=%lld F=%d R=%d out of range", (long long)B, F, R);
+    const int Lw = (mode == HD_SEL_NONE) ? 0 : L;
+    if (Lw < 0 || Lw > 16 || npar < 0 || npar > 7)
+        return set_error(HD_ERR_ARG, "hd_realize_noalloc: L=%d npar=%d invalid", L, npar);
+    const size_t need = gz_ws(B, F, R, Lw, npar);
+    if (need > (size_t)HD_NOALLOC_SCRATCH_BYTES)
+        return set_error(HD_ERR_SIZE,
+                         "hd_realize_noalloc: B=%lld F=%d R=%d L=%d npar=%d needs %zu bytes of scratch, the fixed "
+                         "array holds %zu; use hd_realize with a workspace",
+                         (long long)B, F, R, L, npar, need, (size_t)HD_NOALLOC_SCRATCH_BYTES);
+    void* ws = nullptr;
+    int rc = scratch_address((hipStream_t)stream, &ws);
+    if (rc) return rc;
+    // Nothing in the scratch has to start at zero: every word the kernels read is written
+    // earlier in the same launch sequence (prep: records, counts, sparse axes; draw: every
+    // partial the merges read), so the previous call's contents are never seen.
+    return realize_impl(number, h2fdf, order, B, F, R, L, mode, flags, normal_threshold, seed, par, npar, hc2ss, ssidx,
+                        hc2bg, bgpar, nullptr, ws, (size_t)HD_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 extern "C" int hd_poisson_window(double lam, int cap, int* klo, int* n) {

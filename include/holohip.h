@@ -10,7 +10,8 @@
  *     a message for the calling thread;
  *   - no entry point allocates memory visible to the caller; entry points that
  *     need scratch take a caller-provided workspace whose size is reported by
- *     the matching *_workspace_size() query;
+ *     the matching *_workspace_size() query (hd_realize_noalloc instead uses a
+ *     fixed device array of the library);
  *   - FP64 in and out, as the reference's typed memoryviews (double[:,...]).
  *
  * Each declaration cites the reference interface it replaces (holodeck v1.6,
@@ -226,6 +227,22 @@ int hd_realize_record_counts(const double *number, const double *h2fdf, const in
                              int R, int L, int mode, int flags, double normal_threshold, uint64_t seed,
                              const double *par, int npar, double *hc2ss, int64_t *ssidx, double *hc2bg,
                              double *bgpar, double *counts_out, void *workspace, size_t ws_bytes, void *stream);
+
+/* hd_realize without a workspace, for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  It has no counterpart in
+ * the reference.  Same arguments as hd_realize minus the workspace; same launch sequence, random
+ * streams and outputs for the same seed.  Its scratch is a fixed device array of
+ * HD_NOALLOC_SCRATCH_BYTES on each GPU the library runs on, held until the process ends, so it
+ * takes only sizes with hd_realize_workspace_size(B, F, R, L, npar) <= HD_NOALLOC_SCRATCH_BYTES
+ * (about 3.5 M bin-frequency cells without parameters, e.g. 120,000 bins x 20 frequencies);
+ * larger ones return HD_ERR_SIZE.  Every call on one GPU uses that GPU's one array: calls that
+ * overlap in time on one GPU (other streams, other threads) race, and the caller must
+ * serialise them.  Allocates nothing (no hipMalloc, hipFree or allocating library call).  The
+ * stream must belong to the current device. */
+#define HD_NOALLOC_SCRATCH_BYTES (256ull * 1024ull * 1024ull)
+int hd_realize_noalloc(const double *number, const double *h2fdf, const int64_t *order, int64_t B, int F, int R,
+                       int L, int mode, int flags, double normal_threshold, uint64_t seed, const double *par, int npar,
+                       double *hc2ss, int64_t *ssidx, double *hc2bg, double *bgpar, void *stream);
 
 /* Philox4x32-10 block function (host), for known-answer tests of the
  * generator the realization kernels use. */
