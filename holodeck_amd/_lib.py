@@ -60,9 +60,9 @@ SIGNATURES = {
     "hd_philox4x32_10": (None, [_P, _P, _P]),
     "hd_xoshiro128ss": (None, [_P, _I64, _P]),
     "hd_poisson_cdf_table": (_I32, [C.c_double, _I32, _P, _P, _P]),
-    "hd_poisson_window": (_I32, [C.c_double, _I32, _P, _P]),
-    "hd_table_draw_words": (_I32, [_P, _I32, _P, _I64, _P, _P, _P]),
-    "hd_table_draw_pair_words": (_I32, [_P, _I32, _P, _I32, _P, _I64, _P, _P, _P]),
+    "hd_poisson_window": (_I32, [C.c_double, _I32, _P, _P, _P]),
+    "hd_table_draw_words": (_I32, [_P, _I32, _D, _I32, _I32, _P, _I64, _P, _P, _P]),
+    "hd_table_draw_pair_words": (_I32, [_P, _I32, _D, _I32, _I32, _P, _I32, _D, _I32, _I32, _P, _I64, _P, _P, _P]),
     "hd_mlp_gemm_bf16": (_I32, [_P, _I64, _I64, _P, _I64, _I64, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _I64, _P]),
     "hd_mlp_colsum_bf16": (_I32, [_P, _I32, _I32, _P, _P]),
     "hd_mlp_mse": (_I32, [_P, _P, _I64, _P, _P, _P]),
@@ -154,11 +154,11 @@ def timing_read(name):
 
 
 def poisson_window(lam, cap):
-    """(klo, n) of the realization kernels' CDF window for Poisson(lam) with cap table entries (host)."""
-    klo = C.c_int(0)
-    n = C.c_int(0)
-    call("hd_poisson_window", float(lam), int(cap), C.addressof(klo), C.addressof(n))
-    return klo.value, n.value
+    """(kb, kt, n) of the realization kernels' CDF table for Poisson(lam) with cap entries (host):
+    support from kb, table counts kt .. kt + n - 1."""
+    kb, kt, n = C.c_int(0), C.c_int(0), C.c_int(0)
+    call("hd_poisson_window", float(lam), int(cap), C.addressof(kb), C.addressof(kt), C.addressof(n))
+    return kb.value, kt.value, n.value
 
 
 def xoshiro128ss(state, n):

@@ -17,20 +17,25 @@
 //
 // Every bin is drawn R times (once per realization), so instead of a
 // rejection sampler per draw the workgroup builds the bin's CDF once, in LDS,
-// over a window [klo, khi] around lambda, and every realization inverts its
-// uniform by a branch-free binary search.  With b = 56 ln 2 the window is
+// and every realization inverts its uniform by a branch-free binary search.
+// With b = 56 ln 2 the support considered is [klo, khi],
 //   klo = lambda - sqrt(2 b lambda)              (Chernoff lower tail:
 //         P(X <= lambda - x) <= exp(-x^2 / 2 lambda))
 //   khi = lambda + b/3 + sqrt(b^2/9 + 2 b lambda) (Bernstein upper tail:
 //         P(X >= lambda + x) <= exp(-x^2 / (2 (lambda + x/3))))
 // so each dropped tail holds less than 2^-56, below the 2^-64 resolution of the
-// uniforms; a uniform above the table's last value (CDF rounding, ~n ulp)
-// returns khi.  Bins whose window does not fit the LDS table, and lambda above
-// the normal threshold, are drawn by k_gz_big afterwards (PTRS, Hoermann 1993;
+// uniforms.  The table holds only the centre of it, counts kt .. kc with
+// kt + 1 >= lambda - 4 sqrt(lambda) and kc >= lambda + 4 sqrt(lambda) + 1 (about
+// 8 sqrt(lambda) entries instead of 17.6 sqrt(lambda)); its first entry is the
+// whole CDF up to kt (the lower tail is summed, not stored).  A uniform below the
+// first entry or above the last (probability about 1e-4 per draw) is resolved by
+// walking the pmf recurrence from the table's edge, in FP64, down to klo or up to
+// khi.  Bins whose table does not fit the LDS, and lambda above the normal
+// threshold, are drawn by k_gz_big afterwards (PTRS, Hoermann 1993;
 // Normal(lambda, sqrt(lambda)) unfloored as cyutils.pyx:1328-1332), which
 // continues the same partials.  The sparse tile total N uses the same CDF
 // machinery (as a pseudo-record appended to the tile; its lambda is below
-// 512 tau, so its window always fits).
+// 512 tau, so its table always fits).
 //
 // Random numbers: each (tile, frequency, realization) has one xoshiro128**
 // stream seeded by Philox4x32-10 of (tile, f, r, tag 6) under the seed; the
@@ -66,6 +71,7 @@ constexpr int GZ_BATCH_REC = 128;     // at most this many records per LDS batch
 constexpr int GZ_XCDS = 8;            // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr uint64_t GZ_KEY_NONE = ~0ull;
 constexpr double GZ_TAILB = 38.816242111356935;   // 56 ln 2: each dropped tail < 2^-56
+constexpr double GZ_CENTRE = 4.0;                 // half-width of a table in standard deviations
 #ifndef HD_GZ_TAU
 #define HD_GZ_TAU 1.0   // lambda below which a bin joins its tile's sparse Poisson process
 #endif
@@ -74,18 +80,19 @@ constexpr double GZ_TAILB = 38.816242111356935;   // 56 ln 2: each dropped tail 
 struct DRec {
     double lam, h;
     uint64_t word;    // bin << 32 | rank
-    double p0;        // Poisson pmf at klo
-    int32_t klo;      // first count of the CDF window
-    int32_t n;        // window length (0: no table, use the fallback sampler)
-    int32_t off;      // offset of the window in the tile's concatenated tables
-    int32_t pad;
+    double p0;        // Poisson pmf at kb
+    int32_t kb;       // first count of the support (klo): the table build starts here
+    int32_t n;        // table length (0: no table, use the fallback sampler)
+    int32_t off;      // offset of the table in the tile's concatenated tables
+    int32_t kt;       // count of the table's first entry
 };
 
 // header of a record staged in LDS for the draw loop (off relative to the batch's table)
 struct BRec {
     double h;
     uint64_t word;
-    int32_t klo, n, off, pad;
+    double lam;
+    int32_t kt, n, off, kb;
 };
 
 struct GapArgs {
@@ -120,19 +127,31 @@ __device__ __forceinline__ uint64_t gz_hkey(double h) {
     return ~u;
 }
 
-// CDF window of Poisson(lam): [klo, klo + n); n = 0 (no table: the fallback sampler) if it
-// exceeds cap entries, or if its last count would not fit an int (lam near 2^31: the window
-// would then collapse to a single node by rounding, and klo overflow).
-__host__ __device__ __forceinline__ void gz_window(double lam, int cap, int& klo, int& n) {
+// Support [klo, khi] of Poisson(lam) that drops less than 2^-56 on each side (as doubles).
+__host__ __device__ __forceinline__ void gz_support(double lam, double& lo, double& hi) {
     const double b = GZ_TAILB;
-    const double lo = fmax(floor(lam - sqrt(2.0 * b * lam)), 0.0);
-    const double hi = ceil(lam + b / 3.0 + sqrt(b * b / 9.0 + 2.0 * b * lam));
-    const double w = hi - lo + 1.0;
+    lo = fmax(floor(lam - sqrt(2.0 * b * lam)), 0.0);
+    hi = ceil(lam + b / 3.0 + sqrt(b * b / 9.0 + 2.0 * b * lam));
+}
+
+// Table of Poisson(lam): the build starts at kb = klo, the table holds the counts
+// [kt, kt + n) (entry 0 = the CDF up to kt), kt >= kb and kt + n - 1 <= khi.  n = 0
+// (no table: the fallback sampler) if it exceeds cap entries, or if khi would not
+// fit an int (lam near 2^31: the support would then collapse to a single node by
+// rounding, and the counts overflow).
+__host__ __device__ __forceinline__ void gz_window(double lam, int cap, int& kb, int& kt, int& n) {
+    double lo, hi;
+    gz_support(lam, lo, hi);
+    const double sd = sqrt(lam);
+    const double clo = fmax(floor(lam - GZ_CENTRE * sd), lo);           // first count of the centre
+    const double chi = fmin(ceil(lam + GZ_CENTRE * sd) + 1.0, hi);      // last count of the table
+    const double t0 = fmax(clo - 1.0, lo);                              // entry 0: the CDF below the centre
+    const double w = chi - t0 + 1.0;
     if (!(w <= (double)cap) || !(hi < 2147483647.0)) {
-        klo = 0;
-        n = 0;
+        kb = kt = n = 0;
     } else {
-        klo = (int)lo;
+        kb = (int)lo;
+        kt = (int)t0;
         n = (int)w;
     }
 }
@@ -206,8 +225,8 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
     }
     const bool dense = in && (lam >= A.tau);
     const bool sparse = in && !dense;
-    int klo = 0, nwin = 0;
-    if (dense && lam <= A.thresh) gz_window(lam, A.cap, klo, nwin);
+    int kb = 0, kt = 0, nwin = 0;
+    if (dense && lam <= A.thresh) gz_window(lam, A.cap, kb, kt, nwin);
     const uint64_t md = __ballot(dense), ms = __ballot(sparse), mb = __ballot(dense && nwin == 0);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     double x = sparse ? lam : 0.0;
@@ -256,11 +275,11 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
         r.lam = lam;
         r.h = h;
         r.word = word;
-        r.klo = klo;
+        r.kb = kb;
+        r.kt = kt;
         r.n = nwin;
-        r.p0 = (nwin > 0) ? gz_pmf_acc(lam, (double)klo) : 0.0;
-        r.off = on + xn - nwin;   // exclusive prefix of the window lengths
-        r.pad = 0;
+        r.p0 = (nwin > 0) ? gz_pmf_acc(lam, (double)kb) : 0.0;
+        r.off = on + xn - nwin;   // exclusive prefix of the table lengths
         A.drec[rreg + i] = r;
         A.rec_end[rreg + i] = r.off + nwin;
         for (int k = 0; k < A.npar; k++)
@@ -279,10 +298,9 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
             r.lam = cum;
             r.h = 0.0;
             r.word = 0;
-            gz_window(cum, A.cap, r.klo, r.n);
-            r.p0 = (r.n > 0) ? gz_pmf_acc(cum, (double)r.klo) : 0.0;
+            gz_window(cum, A.cap, r.kb, r.kt, r.n);
+            r.p0 = (r.n > 0) ? gz_pmf_acc(cum, (double)r.kb) : 0.0;
             r.off = ntot;
-            r.pad = 0;
             A.drec[rreg + nd] = r;
             A.rec_end[rreg + nd] = ntot + r.n;
         }
@@ -499,37 +517,38 @@ struct GzMul { __device__ double operator()(double a, double b) const { return a
 struct GzAdd { __device__ double operator()(double a, double b) const { return a + b; } };
 struct GzMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
 
-// Build the CDF window of one record with one wave, as 64-bit fixed point:
-// hi[0, n) the high words, hi[n, 2n) the low words.  Each lane takes a
-// contiguous segment; p(klo) = p0 comes from the prep kernel, and the pmf
-// along the window follows p(k+1) = p(k) lam / (k+1): the lanes' segment
-// products of these ratios are joined by a wave prefix product, each lane
-// sums its segment's recursion, a wave prefix sum gives the carries, and a
-// second run of the recursion writes carry + partial sums.  (Relative error
-// of the CDF about n ulp, ~1e-12 for the widest window, then rounding down to
-// a multiple of 2^-64.)
-__device__ __noinline__ void gz_build(uint32_t* hi, double lam, int klo, int n, double p0, int lane) {
-    uint32_t* lo = hi + n;
-    const int s = (n + 63) >> 6;
+// Build the CDF table of one record with one wave, as 64-bit fixed point:
+// hi[0, n) the high words, hi[n, 2n) the low words, entry j = floor(2^64 CDF(kt + j))
+// with the CDF summed from kb.  The counts kb .. kt + n - 1 are split into contiguous
+// lane segments; p(kb) = p0 comes from the prep kernel and the pmf follows
+// p(k+1) = p(k) lam / (k+1).  Pass 1: each lane's pmf relative to its segment's
+// first count, their product and their sum; a wave prefix product gives each
+// segment's first pmf, a wave prefix sum of the segment sums the carries.  Pass 2
+// repeats the relative sums and writes carry + first pmf * relative sum for the
+// counts from kt on (lanes wholly below kt write nothing: the lower tail is only
+// summed).  (Relative error of the CDF about (kt + n - kb) ulp, then rounding down
+// to a multiple of 2^-64.)
+typedef __attribute__((address_space(3))) uint32_t gz_lds_u32;
+__device__ __noinline__ void gz_build(gz_lds_u32* hi, double lam, int kb, int kt, int n, double p0, int lane) {
+    gz_lds_u32* lo = hi + n;
+    const int nb = kt + n - kb;          // counts summed
+    const int s = (nb + 63) >> 6;
     const int j0 = lane * s;
-    const int m = min(max(n - j0, 0), s);
-    const double kb = (double)(klo + j0);
-    double prod = 1.0;
-    for (int j = 0; j < m; j++) prod *= lam * gz_rcp(kb + (double)(j + 1));
+    const int m = min(max(nb - j0, 0), s);
+    const double kf = (double)(kb + j0);
+    double prod = 1.0, rel = 0.0;        // pass 1
+    for (int j = 0; j < m; j++) {
+        rel += prod;
+        prod *= lam * gz_rcp(kf + (double)(j + 1));
+    }
     const double px = gz_wave_scan(prod, lane, GzMul());
     double pre = __shfl_up(px, 1, 64);   // product over the lanes before this one
     if (lane == 0) pre = 1.0;
-    const double pstart = p0 * pre;      // p(klo + j0)
-    double local = 0.0;
-    {
-        double p = pstart;
-        for (int j = 0; j < m; j++) {
-            local += p;
-            p *= lam * gz_rcp(kb + (double)(j + 1));
-        }
-    }
+    const double pstart = p0 * pre;      // p(kb + j0)
+    const double local = pstart * rel;
     const double x = gz_wave_scan(local, lane, GzAdd());
-    const double carry = (lane > 0) ? x - local : 0.0;
+    double carry = __shfl_up(x, 1, 64);  // exclusive prefix: the sum over the lanes before this one
+    if (lane == 0) carry = 0.0;
     const double last = (m > 0) ? local + carry : -1.0;
     // The carries are rounded independently of the segment sums, so two
     // neighbouring segments can meet out of order by an ulp; raise each
@@ -538,30 +557,37 @@ __device__ __noinline__ void gz_build(uint32_t* hi, double lam, int klo, int n, 
     const double mx = gz_wave_scan(last, lane, GzMax());
     double before = __shfl_up(mx, 1, 64);
     if (lane == 0) before = 0.0;
-    double p = pstart, acc = 0.0;
-    for (int j = 0; j < m; j++) {
-        acc += p;
-        uint32_t h, l;
-        gz_fixed64(fmax(acc + carry, before), h, l);
-        hi[j0 + j] = h;
-        lo[j0 + j] = l;
-        p *= lam * gz_rcp(kb + (double)(j + 1));
+    // pass 2, for segments that reach the table; the same relative sums, so a
+    // segment's last value is exactly `last`
+    const int jt = kt - kb - j0;         // segment index of the table's first count
+    if (jt < m) {
+        double q = 1.0, r = 0.0;
+        for (int j = 0; j < m; j++) {
+            r += q;
+            if (j >= jt) {
+                uint32_t h, l;
+                gz_fixed64(fmax(pstart * r + carry, before), h, l);
+                hi[j0 + j - (kt - kb)] = h;
+                lo[j0 + j - (kt - kb)] = l;
+            }
+            q *= lam * gz_rcp(kf + (double)(j + 1));
+        }
     }
 }
 
-// Inversion by the stream: the smallest k in the window with CDF(k) > u,
-// u = (x, w) / 2^64 for the next words x, w.  A branch-free binary search over
-// the high words finds the first k with hi[k] >= x; when that high word equals
-// x the low words decide, so only then is w taken (probability about n 2^-32
-// per draw).  A uniform above the table's last value (CDF rounding near 1)
-// returns the window's end.  The result is the window index (0 .. n-1);
-// hi[n] (the first low word) is read but not used.  (The candidate is carried
-// as an address, so a step costs one add, one compare and one select, on
-// 4-byte words.)  Two searches run interleaved (gz_table_draw32x2) so their
-// LDS round trips overlap.
+// Inversion by the stream: the smallest count k with CDF(k) > u, u = (x, w) / 2^64
+// for the next words x, w.  A branch-free binary search over the table's high
+// words finds the first entry j with hi[j] >= x; when that high word equals x the
+// low words decide, so only then is w taken (probability about n 2^-32 per
+// draw).  j runs 0 .. n (n: u at or above the last entry); j = 0 with a summed
+// lower tail (kt > kb) or j = n go to the tail walk (which takes w if the search
+// did not), else the count is kt + j.  hi[n] (the first low word) may be read by
+// the search but is not used.  (The candidate is carried as an address, so a
+// step costs one add, one compare and one select, on 4-byte words.)  Two searches
+// run interleaved (gz_draw2) so their LDS round trips overlap.
 struct GzSearch {
     const uint32_t* hi;
-    const uint32_t* p;   // invariant: the first k with hi[k] >= x is in [p - hi, p - hi + len]
+    const uint32_t* p;   // invariant: the first j with hi[j] >= x is in [p - hi, p - hi + len]
     int n, len;
     uint32_t x;
 };
@@ -581,37 +607,104 @@ __device__ __forceinline__ void gz_search_step(GzSearch& s) {
     s.len -= half;
 }
 
+// First entry j (0 .. n) with (hi[j], lo[j]) > (x, w); w is taken only on a tie.
 template <class Words>
-__device__ __forceinline__ int gz_search_finish(const GzSearch& s, Words& rs) {
+__device__ __forceinline__ int gz_search_finish(const GzSearch& s, Words& rs, uint32_t& w, bool& has_w) {
     const uint32_t h0 = s.p[0], h1 = s.p[1];
     const bool up = h0 < s.x;
-    int k = (int)(s.p - s.hi) + (up ? 1 : 0);
-    if (k < s.n && (up ? h1 : h0) == s.x) {
-        const uint32_t w = rs.next();
+    int j = (int)(s.p - s.hi) + (up ? 1 : 0);
+    has_w = false;
+    if (j < s.n && (up ? h1 : h0) == s.x) {
+        w = rs.next();
+        has_w = true;
         const uint32_t* lo = s.hi + s.n;
-        while (k < s.n - 1 && s.hi[k] == s.x && lo[k] <= w) k++;
+        while (j < s.n && s.hi[j] == s.x && lo[j] <= w) j++;
     }
-    return min(k, s.n - 1);
+    return j;
 }
 
+// The count for u = (x, w) / 2^64 outside the table (j = 0 below its first entry,
+// j = n at or above its last), by the pmf recurrence from the table's edge in
+// units of 2^-64: downwards p(k-1) = p(k) k / lam to the support's first count
+// kb, upwards p(k+1) = p(k) lam / (k+1) to its last, khi (the CDF values there
+// differ from u by less than 2^-52 of the tail mass left, which is below 2^-12
+// of the whole).  Each edge pmf is the saddle-point value (relative error
+// ~1e-15).
+__device__ __noinline__ int gz_tail(const gz_lds_u32* hi, int n, int j, uint32_t x, uint32_t w, double lam, int kb,
+                                    int kt) {
+    const gz_lds_u32* lo = hi + n;
+    const uint64_t u = ((uint64_t)x << 32) | w;
+    constexpr double two64 = 18446744073709551616.0;
+    if (j == 0) {   // CDF(kt) > u: the largest k with CDF(k - 1) <= u
+        const uint64_t f0 = ((uint64_t)hi[0] << 32) | lo[0];
+        double d = (double)(f0 - u);                        // CDF(k) - u, k = kt
+        double p = gz_pmf_acc(lam, (double)kt) * two64;     // p(k)
+        int k = kt;
+        while (k > kb && d > p) {
+            d -= p;
+            p *= (double)k / lam;
+            k--;
+        }
+        return k;
+    }
+    const int kc = kt + n - 1;   // CDF(kc) <= u: the first k > kc with CDF(k) > u
+    double slo, shi;
+    gz_support(lam, slo, shi);
+    const int khi = (int)shi;
+    if (kc >= khi) return kc;    // above the support's CDF only by rounding
+    const uint64_t fl = ((uint64_t)hi[n - 1] << 32) | lo[n - 1];
+    double d = (double)(u - fl);                             // u - CDF(k - 1), k = kc + 1
+    int k = kc + 1;
+    double p = gz_pmf_acc(lam, (double)k) * two64;           // p(k)
+    while (k < khi && d >= p) {
+        d -= p;
+        k++;
+        p *= lam / (double)k;
+    }
+    return k;
+}
+
+// A table as the draws see it: its words, length, first counts, and the staged
+// header (lam is read from there only by a tail walk).
+struct GzTab {
+    const uint32_t* hi;
+    int n, kb, kt;
+    const BRec* rec;
+};
+
 template <class Words>
-__device__ __forceinline__ int gz_table_draw32(const uint32_t* hi, int n, Words& rs) {
+__device__ __forceinline__ int gz_count(const GzTab& t, int j, uint32_t x, uint32_t w, bool has_w, Words& rs) {
+    if ((j == 0 && t.kt > t.kb) || j == t.n) {
+        if (!has_w) w = rs.next();
+        return gz_tail((const gz_lds_u32*)t.hi, t.n, j, x, w, t.rec->lam, t.kb, t.kt);
+    }
+    return t.kt + j;
+}
+
+// One draw from a table (count kt + j of its entries, tails beyond).
+template <class Words>
+__device__ __forceinline__ int gz_draw1(const GzTab& t, Words& rs) {
     GzSearch s;
-    gz_search_start(s, hi, n, rs.next());
+    const uint32_t x = rs.next();
+    gz_search_start(s, t.hi, t.n, x);
     while (s.len > 1) gz_search_step(s);
-    return gz_search_finish(s, rs);
+    uint32_t w = 0;
+    bool hw;
+    const int j = gz_search_finish(s, rs, w, hw);
+    return gz_count(t, j, x, w, hw, rs);
 }
 
-// Two draws from tables a and b (n = 0: no draw).  Words in the order x_a, x_b,
-// then w_a, w_b where needed.
+// Two draws from tables a and b (n = 0: no draw, count 0).  Words in the order
+// x_a, x_b, then w_a, w_b where a tie needs them, then w_a, w_b where a tail
+// walk needs them and the search did not take them.
+
 template <class Words>
-__device__ __forceinline__ void gz_table_draw32x2(const uint32_t* ha, int na, const uint32_t* hb, int nb, Words& rs,
-                                                  int& ka, int& kb) {
+__device__ __forceinline__ void gz_draw2(const GzTab& ta, const GzTab& tb, Words& rs, int& ca, int& cb) {
     GzSearch a, b;
-    const uint32_t xa = (na > 0) ? rs.next() : 0u;
-    const uint32_t xb = (nb > 0) ? rs.next() : 0u;
-    gz_search_start(a, ha, na, xa);
-    gz_search_start(b, hb, nb, xb);
+    const uint32_t xa = (ta.n > 0) ? rs.next() : 0u;
+    const uint32_t xb = (tb.n > 0) ? rs.next() : 0u;
+    gz_search_start(a, ta.hi, ta.n, xa);
+    gz_search_start(b, tb.hi, tb.n, xb);
     while (a.len > 1 && b.len > 1) {
         // both loads issued before either compare, so their LDS latencies overlap
         const int ha = a.len >> 1, hb = b.len >> 1;
@@ -626,8 +719,24 @@ __device__ __forceinline__ void gz_table_draw32x2(const uint32_t* ha, int na, co
     }
     while (a.len > 1) gz_search_step(a);
     while (b.len > 1) gz_search_step(b);
-    ka = (na > 0) ? gz_search_finish(a, rs) : 0;
-    kb = (nb > 0) ? gz_search_finish(b, rs) : 0;
+    uint32_t wa = 0, wb = 0;
+    bool hwa = false, hwb = false;
+    const int ja = (ta.n > 0) ? gz_search_finish(a, rs, wa, hwa) : 1;
+    const int jb = (tb.n > 0) ? gz_search_finish(b, rs, wb, hwb) : 1;
+    ca = (ta.n > 0) ? gz_count(ta, ja, xa, wa, hwa, rs) : 0;
+    cb = (tb.n > 0) ? gz_count(tb, jb, xb, wb, hwb, rs) : 0;
+}
+
+// The table of a staged header (the header is the same for the whole wave: its
+// fields go to scalar registers); present = false gives an empty table.
+__device__ __forceinline__ GzTab gz_tab(const uint32_t* tab, const BRec* b, bool present) {
+    GzTab t;
+    t.n = present ? __builtin_amdgcn_readfirstlane(b->n) : 0;
+    t.hi = tab + 2 * __builtin_amdgcn_readfirstlane(b->off);
+    t.kb = __builtin_amdgcn_readfirstlane(b->kb);
+    t.kt = __builtin_amdgcn_readfirstlane(b->kt);
+    t.rec = b;
+    return t;
 }
 
 // Counts without a table: Normal above the threshold, inversion below 10,
@@ -726,8 +835,10 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
                         inext = __shfl(inext, 0, 64);
                         DRec nrec;
                         if (inext < c1) nrec = A.drec[rreg + inext];
-                        if (lane == 0) brec[i - c] = BRec{rec.h, rec.word, rec.klo, rec.n, rec.off - base, 0};
-                        if (rec.n > 0) gz_build(tab + 2 * (rec.off - base), rec.lam, rec.klo, rec.n, rec.p0, lane);
+                        if (lane == 0) brec[i - c] = BRec{rec.h, rec.word, rec.lam, rec.kt, rec.n, rec.off - base, rec.kb};
+                        if (rec.n > 0)
+                            gz_build((gz_lds_u32*)(tab + 2 * (rec.off - base)), rec.lam, rec.kb, rec.kt, rec.n, rec.p0,
+                                     lane);
                         i = inext;
                         rec = nrec;
                     }
@@ -741,28 +852,27 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
                     // (records in pairs, their searches interleaved; n = 0: no table, drawn by k_gz_big)
                     const int cd = min(c1, nd);
                     for (int i = c; i < cd; i += 2) {
-                        const BRec ra = brec[i - c];
+                        const BRec* pa = brec + (i - c);
                         const bool hb = i + 1 < cd;
-                        const BRec rb = brec[hb ? i + 1 - c : i - c];
-                        const int na = ra.n, nb = hb ? rb.n : 0;
+                        const BRec* pb = brec + (hb ? i + 1 - c : i - c);
+                        const GzTab ta = gz_tab(tab, pa, true), tb = gz_tab(tab, pb, hb);
                         int ka, kb;
-                        gz_table_draw32x2(tab + 2 * ra.off, na, tab + 2 * rb.off, nb, rs, ka, kb);
-                        if (na > 0) acc.add((double)(ra.klo + ka), ra.h, ra.word, A.d_par + reg + i, pstride, A.npar, L);
-                        if (nb > 0)
-                            acc.add((double)(rb.klo + kb), rb.h, rb.word, A.d_par + reg + i + 1, pstride, A.npar, L);
+                        gz_draw2(ta, tb, rs, ka, kb);
+                        if (ta.n > 0) acc.add((double)ka, pa->h, pa->word, A.d_par + reg + i, pstride, A.npar, L);
+                        if (tb.n > 0) acc.add((double)kb, pb->h, pb->word, A.d_par + reg + i + 1, pstride, A.npar, L);
                         if (REC) {
-                            if (na > 0) gz_record(A, ra.word, f, r, (double)(ra.klo + ka), false);
-                            if (nb > 0) gz_record(A, rb.word, f, r, (double)(rb.klo + kb), false);
+                            if (ta.n > 0) gz_record(A, pa->word, f, r, (double)ka, false);
+                            if (tb.n > 0) gz_record(A, pb->word, f, r, (double)kb, false);
                         }
                     }
                     for (int i = max(c, nd); i < c1; i++) {
-                        const BRec rec = brec[i - c];
+                        const BRec* rec = brec + (i - c);
                         {
                             // sparse bins: Poisson process on the tile's cumulative-lambda axis; the
-                            // point count from its table (the window always fits: lambda < 512), the
+                            // point count from its table (it always fits: lambda < 512), the
                             // positions uniform on the axis (53-bit uniforms)
                             const double Lam = scum[ns - 1];
-                            const int np = rec.klo + gz_table_draw32(tab + 2 * rec.off, rec.n, rs);
+                            const int np = gz_draw1(gz_tab(tab, rec, true), rs);
                             for (int q = 0; q < np; q++) {
                                 const uint32_t w0 = rs.next();
                                 const double xq = u53(w0, rs.next()) * Lam;
@@ -833,14 +943,15 @@ __global__ void __launch_bounds__(64) k_cdf_table_hook(double lam, int cap, int*
     extern __shared__ double hook_lds[];
     uint32_t* tab = (uint32_t*)hook_lds;
     const int lane = threadIdx.x;
-    int klo, n;
-    gz_window(lam, cap, klo, n);
-    if (n > 0) gz_build(tab, lam, klo, n, gz_pmf_acc(lam, (double)klo), lane);
+    int kb, kt, n;
+    gz_window(lam, cap, kb, kt, n);
+    if (n > 0) gz_build((gz_lds_u32*)tab, lam, kb, kt, n, gz_pmf_acc(lam, (double)kb), lane);
     __syncthreads();
     for (int j = lane; j < n; j += 64) cdf_out[j] = ((uint64_t)tab[j] << 32) | tab[n + j];
     if (lane == 0) {
-        kn[0] = klo;
-        kn[1] = n;
+        kn[0] = kb;
+        kn[1] = kt;
+        kn[2] = n;
     }
 }
 
@@ -850,23 +961,45 @@ struct GzWords {   // a stream that replays given words
     __device__ uint32_t next() { return w[i++]; }
 };
 
-__global__ void k_table_draw_hook(const uint32_t* tab, int n, const uint32_t* words, int64_t m, int* slot, int* used) {
+// Draws from given tables (copied into LDS as the kernels hold them) with given
+// words: 2 words per draw, or 4 per pair of draws.
+__global__ void __launch_bounds__(256) k_table_draw_hook(const uint32_t* tab, int n, double lam, int kb, int kt,
+                                                         const uint32_t* words, int64_t m, int* count, int* used) {
+    extern __shared__ double hook_lds[];
+    uint32_t* t = (uint32_t*)hook_lds;
+    __shared__ BRec hdr;
+    for (int j = threadIdx.x; j < 2 * n; j += blockDim.x) t[j] = tab[j];
+    if (threadIdx.x == 0) hdr = BRec{0.0, 0, lam, kt, n, 0, kb};
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     GzWords ws{words + 2 * i, 0};
-    slot[i] = gz_table_draw32(tab, n, ws);
+    count[i] = gz_draw1(GzTab{t, n, kb, kt, &hdr}, ws);
     used[i] = ws.i;
 }
 
-__global__ void k_table_draw_pair_hook(const uint32_t* tab, int n, const uint32_t* tab2, int n2, const uint32_t* words,
-                                       int64_t m, int* slot, int* used) {
+__global__ void __launch_bounds__(256) k_table_draw_pair_hook(const uint32_t* tab, int n, double lam, int kb, int kt,
+                                                              const uint32_t* tab2, int n2, double lam2, int kb2,
+                                                              int kt2, const uint32_t* words, int64_t m, int* count,
+                                                              int* used) {
+    extern __shared__ double hook_lds[];
+    uint32_t* t = (uint32_t*)hook_lds;
+    uint32_t* t2 = t + 2 * n;
+    for (int j = threadIdx.x; j < 2 * n; j += blockDim.x) t[j] = tab[j];
+    __shared__ BRec hdr[2];
+    for (int j = threadIdx.x; j < 2 * n2; j += blockDim.x) t2[j] = tab2[j];
+    if (threadIdx.x == 0) {
+        hdr[0] = BRec{0.0, 0, lam, kt, n, 0, kb};
+        hdr[1] = BRec{0.0, 0, lam2, kt2, n2, 0, kb2};
+    }
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     GzWords ws{words + 4 * i, 0};
-    int ka, kb;
-    gz_table_draw32x2(tab, n, tab2, n2, ws, ka, kb);
-    slot[2 * i] = ka;
-    slot[2 * i + 1] = kb;
+    int ca, cb;
+    gz_draw2(GzTab{t, n, kb, kt, &hdr[0]}, GzTab{t2, n2, kb2, kt2, &hdr[1]}, ws, ca, cb);
+    count[2 * i] = ca;
+    count[2 * i + 1] = cb;
     used[i] = ws.i;
 }
 
@@ -879,14 +1012,16 @@ int rank_from_order_launch(const int64_t* order, int64_t B, int32_t* rank, hipSt
 
 static size_t gz_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static int gz_block(int R) { return (int)min((int64_t)GZ_MAXBS, (int64_t)cdiv(R, 64) * 64); }
+// Draw workgroups always have GZ_MAXBS threads: with fewer realizations the
+// threads beyond R only build tables (eight waves share the builds at any R).
+static int gz_block(int) { return GZ_MAXBS; }
 // LDS table capacity (8-byte slots) of every draw workgroup: table + sparse
-// axis + staged record headers + the build counter stay within 53 KB, so that
+// axis + staged record headers + the build counter stay within 50 KB, so that
 // three workgroups fit one CU's 160 KB (allocation granularity included).
 // Fixed, so that which bins get a table (and so every draw) does not depend
 // on the launch shape.
 #ifndef HD_GZ_CAP
-#define HD_GZ_CAP 5632
+#define HD_GZ_CAP 5120
 #endif
 static int gz_cap(int) { return HD_GZ_CAP; }
 // draw workgroup LDS: table + sparse axis + staged headers (+ the build counter); three must fit a
@@ -1131,32 +1266,40 @@ extern "C" int hd_realize_noalloc(const double* number, const double* h2fdf, con
                         hc2bg, bgpar, nullptr, ws, (size_t)HD_NOALLOC_SCRATCH_BYTES, stream);
 }
 
-extern "C" int hd_poisson_window(double lam, int cap, int* klo, int* n) {
-    if (!(lam > 0.0) || !(lam < 1e300) || cap < 1 || !klo || !n) return set_error(HD_ERR_ARG, "hd_poisson_window: bad arguments");
-    gz_window(lam, cap, *klo, *n);
+extern "C" int hd_poisson_window(double lam, int cap, int* kb, int* kt, int* n) {
+    if (!(lam > 0.0) || !(lam < 1e300) || cap < 1 || !kb || !kt || !n)
+        return set_error(HD_ERR_ARG, "hd_poisson_window: bad arguments");
+    gz_window(lam, cap, *kb, *kt, *n);
     return HD_OK;
 }
 
-extern "C" int hd_poisson_cdf_table(double lam, int cap, int* klo_n, uint64_t* cdf, void* stream) {
-    if (!(lam > 0.0) || !(lam < 1e300) || cap < 1 || cap > GZ_HOOK_CAP_MAX || !klo_n || !cdf)
+extern "C" int hd_poisson_cdf_table(double lam, int cap, int* kb_kt_n, uint64_t* cdf, void* stream) {
+    if (!(lam > 0.0) || !(lam < 1e300) || cap < 1 || cap > GZ_HOOK_CAP_MAX || !kb_kt_n || !cdf)
         return set_error(HD_ERR_ARG, "hd_poisson_cdf_table: bad arguments");
-    {   // the window is checked on the host first: a table only when it fits
-        int klo, n;
-        gz_window(lam, cap, klo, n);
-        if (n < 1) return set_error(HD_ERR_SIZE, "hd_poisson_cdf_table: window of lam=%g exceeds cap=%d", lam, cap);
+    {   // the table is sized on the host first: a launch only when it fits
+        int kb, kt, n;
+        gz_window(lam, cap, kb, kt, n);
+        if (n < 1) return set_error(HD_ERR_SIZE, "hd_poisson_cdf_table: table of lam=%g exceeds cap=%d", lam, cap);
     }
     hipLaunchKernelGGL(k_cdf_table_hook, dim3(1), dim3(64), (size_t)cap * sizeof(double), (hipStream_t)stream, lam,
-                       cap, klo_n, cdf);
+                       cap, kb_kt_n, cdf);
     return check_launch("k_cdf_table_hook");
 }
 
-extern "C" int hd_table_draw_words(const uint32_t* tab, int n, const uint32_t* words, int64_t m, int* slot, int* used,
-                                   void* stream) {
-    if (n < 1 || m < 0 || !tab || (m > 0 && (!words || !slot || !used)))
+static bool gz_hook_table_ok(int n, double lam, int kb, int kt) {
+    if (n < 2 || n > GZ_HOOK_CAP_MAX || !(lam > 0.0) || kb < 0 || kt < kb) return false;
+    int b, t, w;   // the table must be the one the kernels build for lam (its tails walk from there)
+    gz_window(lam, GZ_HOOK_CAP_MAX, b, t, w);
+    return w == n && b == kb && t == kt;
+}
+
+extern "C" int hd_table_draw_words(const uint32_t* tab, int n, double lam, int kb, int kt, const uint32_t* words,
+                                   int64_t m, int* count, int* used, void* stream) {
+    if (!tab || !gz_hook_table_ok(n, lam, kb, kt) || m < 0 || (m > 0 && (!words || !count || !used)))
         return set_error(HD_ERR_ARG, "hd_table_draw_words: bad arguments");
     if (m == 0) return HD_OK;
-    hipLaunchKernelGGL(k_table_draw_hook, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, (hipStream_t)stream, tab, n,
-                       words, m, slot, used);
+    hipLaunchKernelGGL(k_table_draw_hook, dim3((unsigned)cdiv(m, 256)), dim3(256), (size_t)n * 2 * sizeof(uint32_t),
+                       (hipStream_t)stream, tab, n, lam, kb, kt, words, m, count, used);
     return check_launch("k_table_draw_hook");
 }
 
@@ -1176,12 +1319,15 @@ extern "C" void hd_xoshiro128ss(uint32_t* state, int64_t n, uint32_t* out) {
     state[3] = g.s3;
 }
 
-extern "C" int hd_table_draw_pair_words(const uint32_t* tab, int n, const uint32_t* tab2, int n2,
-                                        const uint32_t* words, int64_t m, int* slot, int* used, void* stream) {
-    if (n < 1 || n2 < 1 || m < 0 || !tab || !tab2 || (m > 0 && (!words || !slot || !used)))
+extern "C" int hd_table_draw_pair_words(const uint32_t* tab, int n, double lam, int kb, int kt, const uint32_t* tab2,
+                                        int n2, double lam2, int kb2, int kt2, const uint32_t* words, int64_t m,
+                                        int* count, int* used, void* stream) {
+    if (!tab || !tab2 || !gz_hook_table_ok(n, lam, kb, kt) || !gz_hook_table_ok(n2, lam2, kb2, kt2) ||
+        n + n2 > GZ_HOOK_CAP_MAX || m < 0 || (m > 0 && (!words || !count || !used)))
         return set_error(HD_ERR_ARG, "hd_table_draw_pair_words: bad arguments");
     if (m == 0) return HD_OK;
-    hipLaunchKernelGGL(k_table_draw_pair_hook, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, (hipStream_t)stream, tab,
-                       n, tab2, n2, words, m, slot, used);
+    hipLaunchKernelGGL(k_table_draw_pair_hook, dim3((unsigned)cdiv(m, 256)), dim3(256),
+                       (size_t)(n + n2) * 2 * sizeof(uint32_t), (hipStream_t)stream, tab, n, lam, kb, kt, tab2, n2,
+                       lam2, kb2, kt2, words, m, count, used);
     return check_launch("k_table_draw_pair_hook");
 }

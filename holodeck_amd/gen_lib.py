@@ -6,7 +6,9 @@ Latin-hypercube samples of the classic uniform phenomenological space; with ``--
 sample gets its own grid (edge counts uniform in mtot 41-91, mrat 41-81, redz 41-101 from
 ``--shape-seed``).  Only the independent Poisson GWB is made (``gwb``, (S, F, R)), on the
 pipelined device path; ``--full`` runs ``run_model`` per sample instead (loud sources and
-background too).  The reference's MPI scatter of sample numbers over processes becomes one
+background too).  A sample that fails is marked (``failed``, ``fail_messages``, NaN outputs and
+parameters) and the rest go on; ``--out`` is written after every block and a rerun resumes it
+(``--recreate`` starts over; with ``--full``, ``--sims-dir`` keeps one file per sample).  The reference's MPI scatter of sample numbers over processes becomes one
 process per GPU; this build runs on one GPU.
 """
 import argparse
@@ -33,34 +35,44 @@ def main(argv=None):
     ap.add_argument("--streams", default="auto", help="HIP streams, or 'auto' (timed on the first samples)")
     ap.add_argument("--full", action="store_true", help="run_model per sample (gwb, hc_ss, hc_bg)")
     ap.add_argument("--log-every", type=int, default=256)
-    ap.add_argument("--out", default=None, help="write the library (.npz)")
+    ap.add_argument("--out", default=None,
+                    help="write the library (.npz); it is also the checkpoint a rerun resumes from")
+    ap.add_argument("--recreate", action="store_true", help="recompute everything instead of resuming")
+    ap.add_argument("--sims-dir", default=None,
+                    help="with --full: one file per sample there (resumed as gen_lib.py:283-294)")
     a = ap.parse_args(argv)
     ps = librarian.PS_Classic_Phenom_Uniform(nsamples=a.nsamples, sam_shape=tuple(a.shape), seed=a.seed)
     shapes = librarian.sample_shapes(a.nsamples, a.shape_seed) if a.uneven else None
     t0 = time.perf_counter()
     if a.full:
         lib = librarian.run_library(ps, shapes=shapes, pta_dur=a.pta_dur, nfreqs=a.nfreqs, nreals=a.nreals,
-                                    nloudest=a.nloudest, seed=a.seed, log_every=a.log_every)
+                                    nloudest=a.nloudest, seed=a.seed, log_every=a.log_every, sims_dir=a.sims_dir,
+                                    recreate=a.recreate)
         rate = a.nsamples / (time.perf_counter() - t0)
         used = 1
+        computed = a.nsamples - int(lib['resumed'])
     else:
         streams = a.streams if a.streams == "auto" else int(a.streams)
         lib = librarian.gwb_library(ps, shapes=shapes, pta_dur=a.pta_dur, nfreqs=a.nfreqs, nreals=a.nreals,
-                                    seed=a.seed, streams=streams, log_every=a.log_every)
+                                    seed=a.seed, streams=streams, log_every=a.log_every, path=a.out,
+                                    resume=not a.recreate)
         rate = lib.pop('samples_per_s')
         used = int(lib.pop('streams_used'))
+        computed = int(a.nsamples)
     if shapes is not None:
         lib['sam_shapes'] = shapes
     cells = (np.prod(shapes - 1, axis=1).sum() if shapes is not None
              else a.nsamples * np.prod(np.array(a.shape) - 1))
     gwb = lib['gwb']
-    ok = bool(np.all(np.isfinite(gwb)) and np.all(gwb >= 0))
+    good = ~lib['failed']
+    ok = bool(np.all(np.isfinite(gwb[good])) and np.all(gwb[good] >= 0))
     if a.out:
         np.savez(a.out, **lib)
-    print(json.dumps({"samples": a.nsamples, "samples_per_s": rate, "uneven": bool(a.uneven),
+    print(json.dumps({"samples": a.nsamples, "computed": computed, "failed": int((~good).sum()),
+                      "samples_per_s": rate, "uneven": bool(a.uneven),
                       "nfreqs": a.nfreqs, "nreals": a.nreals, "full": bool(a.full), "finite": ok, "streams": used,
                       "cell_realizations_per_s": float(cells) * a.nfreqs * a.nreals * rate / a.nsamples,
-                      "median_hc_f0": float(np.median(gwb[:, 0, :]))}), flush=True)
+                      "median_hc_f0": float(np.nanmedian(gwb[:, 0, :]))}), flush=True)
     return 0 if ok else 1
 
 

@@ -312,41 +312,105 @@ def sample_shapes(nsamples, seed, mtot=(41, 91), mrat=(41, 81), redz=(41, 101)):
     return np.stack([rng.integers(lo, hi + 1, size=nsamples) for lo, hi in (mtot, mrat, redz)], axis=1)
 
 
+def sim_fname(sims_dir, snum):
+    """Per-sample output file of a library run (lib_tools._get_sim_fname: ``sim_{snum:06d}.npz``)."""
+    import os
+    return os.path.join(sims_dir, f"sim_{int(snum):06d}.npz")
+
+
+def _sample_done(fname, recreate):
+    """True when ``fname`` holds a finished sample to keep (gen_lib.py:283-294): an existing file
+    without the 'fail' key, unless ``recreate``."""
+    import os
+    if recreate or not os.path.exists(fname):
+        return False
+    try:
+        with np.load(fname, allow_pickle=False) as fh:
+            return 'fail' not in fh.files
+    except (OSError, ValueError):   # an unreadable (e.g. truncated) file is recomputed
+        return False
+
+
 def run_library(pspace, shapes=None, pta_dur=DEF_PTA_DUR, nfreqs=DEF_NUM_FBINS, nreals=DEF_NUM_REALS,
                 nloudest=DEF_NUM_LOUDEST, gwb_flag=True, singles_flag=True, params_flag=False, seed=0,
-                path=None, samples=None, cosmo=None, log_every=0):
+                path=None, samples=None, cosmo=None, log_every=0, sims_dir=None, recreate=False):
     """Run every sample of `pspace` (or the listed sample numbers) and collect the library arrays.
 
     ``shapes``: (S, 3) grid edge counts per sample, or None for ``pspace.sam_shape``.
     Returns the dictionary of the combined library file and writes it to ``path`` (``.npz``) if
     given.  Sample ``i`` uses the Philox seed ``seed + 2 i``.
+
+    Failures are isolated per sample (gen_lib.py:287-320, combine.py:206-212): an exception in a
+    sample's model or run, or a non-finite output, marks that sample failed -- its outputs are NaN,
+    its ``sample_params`` row NaN, ``failed[i]`` True and ``fail_messages[i]`` the reason -- and the
+    other samples go on.  With ``sims_dir`` every sample is also written to its own file
+    (``sim_fname``; a failure as a file with the single key 'fail'), and a rerun resumes: samples
+    whose file exists without 'fail' are read back instead of recomputed (all are recomputed with
+    ``recreate``).
     """
+    import os
     samples = np.arange(pspace.nsamples) if samples is None else np.asarray(samples)
     fobs_cents, fobs_edges = utils.pta_freqs(dur=pta_dur * YR, num=nfreqs)
     S = len(samples)
     lib = {'fobs_cents': fobs_cents, 'fobs_edges': fobs_edges,
            'sample_params': pspace.param_samples[samples].copy()}
+    keys = []
     if gwb_flag:
         lib['gwb'] = np.zeros((S, nfreqs, nreals))
+        keys.append('gwb')
     if singles_flag:
         lib['hc_ss'] = np.zeros((S, nfreqs, nreals, nloudest))
         lib['hc_bg'] = np.zeros((S, nfreqs, nreals))
+        keys += ['hc_ss', 'hc_bg']
     if params_flag:
         lib['sspar'] = np.zeros((S, 4, nfreqs, nreals, nloudest))
         lib['bgpar'] = np.zeros((S, 7, nfreqs, nreals))
+        keys += ['sspar', 'bgpar']
+    failed = np.zeros(S, dtype=bool)
+    messages = [''] * S
+    resumed = 0
+    if sims_dir is not None:
+        os.makedirs(sims_dir, exist_ok=True)
     t0 = time.perf_counter()
     for ii, snum in enumerate(samples):
+        fname = None if sims_dir is None else sim_fname(sims_dir, snum)
+        if fname is not None and _sample_done(fname, recreate):
+            with np.load(fname, allow_pickle=False) as fh:
+                for key in keys:
+                    lib[key][ii] = fh[key]
+            resumed += 1
+            continue
         shape = None if shapes is None else tuple(int(v) for v in shapes[ii])
-        sam, hard = pspace.model_for_sample_number(int(snum), sam_shape=shape, cosmo=cosmo)
-        data = run_model(sam, hard, pta_dur=pta_dur, nfreqs=nfreqs, nreals=nreals, nloudest=nloudest,
-                         gwb_flag=gwb_flag, singles_flag=singles_flag, params_flag=params_flag,
-                         seed=seed + 2 * int(snum))
-        for key in ('gwb', 'hc_ss', 'hc_bg', 'sspar', 'bgpar'):
-            if key in lib:
-                lib[key][ii] = data[key]
+        try:
+            sam, hard = pspace.model_for_sample_number(int(snum), sam_shape=shape, cosmo=cosmo)
+            data = run_model(sam, hard, pta_dur=pta_dur, nfreqs=nfreqs, nreals=nreals, nloudest=nloudest,
+                             gwb_flag=gwb_flag, singles_flag=singles_flag, params_flag=params_flag,
+                             seed=seed + 2 * int(snum))
+            bad = [key for key in keys if not np.all(np.isfinite(data[key]))]
+            if bad:
+                raise FloatingPointError(f"non-finite values in {bad}")
+        except Exception as err:   # the sample fails, the library goes on
+            failed[ii] = True
+            messages[ii] = f"{type(err).__name__}: {err}"
+            for key in keys:
+                lib[key][ii] = np.nan
+            if fname is not None:
+                np.savez(fname, fail=np.array(messages[ii]))
+            continue
+        for key in keys:
+            lib[key][ii] = data[key]
+        if fname is not None:
+            out = {key: data[key] for key in keys}
+            out['params'] = pspace.param_samples[int(snum)]
+            out['param_names'] = np.array(pspace.param_names).astype('S')
+            np.savez(fname, **out)
         if log_every and (ii + 1) % log_every == 0:
             dt = time.perf_counter() - t0
             print(f"library: {ii + 1}/{S} samples, {dt:.1f} s ({(ii + 1) / dt:.2f} samples/s)", flush=True)
+    lib['sample_params'][failed] = np.nan
+    lib['failed'] = failed
+    lib['fail_messages'] = np.array(messages)
+    lib['resumed'] = np.array(resumed)
     lib['param_names'] = np.array(pspace.param_names).astype('S')
     lib['parameter_space_class_name'] = np.array(pspace.name)
     if path is not None:
@@ -356,29 +420,87 @@ def run_library(pspace, shapes=None, pta_dur=DEF_PTA_DUR, nfreqs=DEF_NUM_FBINS, 
 
 def gwb_library(pspace, shapes=None, pta_dur=DEF_PTA_DUR, nfreqs=DEF_NUM_FBINS, nreals=DEF_NUM_REALS, seed=0,
                 samples=None, cosmo=None, streams=3, chunk=256, reduce=None, reduce_key='targets', plan_workers=8,
-                lookahead=16, log_every=0):
+                lookahead=16, log_every=0, path=None, resume=True):
     """The independent Poisson GWB of every sample (``run_model``'s ``gwb_flag`` output alone),
     with samples in flight on several HIP streams (:func:`gwb_for_params`).
 
     Sample ``i`` uses the Philox seed ``seed + 2 i + 1``, so ``gwb`` equals :func:`run_library`'s
-    for the same ``seed``.  ``reduce``: None keeps ``gwb`` (S, F, R); else a function of one block's
-    (n, F, R) strain whose outputs are concatenated under ``reduce_key`` (e.g. percentile targets)
-    and ``gwb`` is not kept.  Returns the library dictionary plus ``samples_per_s`` (whole run,
-    wall clock).
+    for the same ``seed``.  ``reduce``: None keeps ``gwb`` (S, F, R); else a function mapping a
+    block's (n, F, R) strain to per-sample rows (n, ...), kept under ``reduce_key`` (e.g.
+    percentile targets) instead of ``gwb``.  A sample that fails (:func:`gwb_for_params`) has NaN
+    outputs and ``sample_params``, ``failed`` True and its reason in ``fail_messages``.
+
+    With ``path`` the library is written there after every block of samples (a checkpoint:
+    ``done`` marks the samples finished), and, with ``resume``, an existing file of the same
+    samples, seed, frequencies and realizations is continued: only samples not done (never run,
+    or failed) are computed.  Returns the library dictionary plus ``samples_per_s`` (this call,
+    wall clock, over the samples it computed).
     """
+    import os
     samples = np.arange(pspace.nsamples) if samples is None else np.asarray(samples)
+    S = samples.size
     fobs_cents, fobs_edges = utils.pta_freqs(dur=pta_dur * YR, num=int(nfreqs))
     params = pspace.param_samples[samples]
-    t0 = time.perf_counter()
-    info = {}
-    out = gwb_for_params(pspace, params, shapes=shapes, fobs_edges=fobs_edges, nreals=nreals,
-                         seeds=seed + 2 * samples.astype(np.int64) + 1, cosmo=cosmo, streams=streams, chunk=chunk,
-                         reduce=reduce, plan_workers=plan_workers, lookahead=lookahead, log_every=log_every, info=info)
+    seeds = seed + 2 * samples.astype(np.int64) + 1
+    key = 'gwb' if reduce is None else reduce_key
     lib = {'fobs_cents': fobs_cents, 'fobs_edges': fobs_edges, 'sample_params': params.copy(),
-           ('gwb' if reduce is None else reduce_key): out,
-           'samples_per_s': len(samples) / (time.perf_counter() - t0), 'streams_used': np.array(info['streams']),
+           'samples': samples.copy(), 'seed': np.array(seed), 'nreals': np.array(int(nreals)),
+           'done': np.zeros(S, dtype=bool), 'failed': np.zeros(S, dtype=bool),
+           'fail_messages': np.array([''] * S, dtype=object),
            'param_names': np.array(pspace.param_names).astype('S'),
            'parameter_space_class_name': np.array(pspace.name)}
+    if shapes is not None:
+        lib['sam_shapes'] = np.asarray(shapes)
+    lib[key] = np.full((S, int(nfreqs), int(nreals)), np.nan) if reduce is None else None
+    if path is not None and resume and os.path.exists(path):
+        with np.load(path, allow_pickle=False) as fh:
+            same = (key in fh.files and np.array_equal(fh['samples'], samples) and int(fh['seed']) == int(seed)
+                    and int(fh['nreals']) == int(nreals) and np.array_equal(fh['fobs_edges'], fobs_edges))
+            if not same:
+                raise ValueError(f"{path} holds another library (samples, seed, frequencies or realizations "
+                                 "differ); remove it or pass resume=False")
+            lib['done'] = fh['done'].copy()
+            lib[key] = fh[key].copy()
+            lib['fail_messages'] = fh['fail_messages'].astype(object)
+    todo = np.flatnonzero(~lib['done'])
+    lib['failed'][todo] = False
+    lib['fail_messages'][todo] = ''
+
+    def save():
+        out = dict(lib)
+        out['sample_params'] = np.where(lib['failed'][:, None], np.nan, lib['sample_params'])
+        out['fail_messages'] = lib['fail_messages'].astype(str)
+        tmp = path + ".tmp.npz"
+        np.savez(tmp, **out)
+        os.replace(tmp, path)
+
+    def on_block(start, hc, fl, ms):
+        rows = todo[start:start + hc.shape[0]]
+        vals = hc if reduce is None else np.asarray(reduce(hc))
+        if lib[key] is None:
+            lib[key] = np.full((S,) + vals.shape[1:], np.nan)
+        vals = np.array(vals, dtype=float)
+        vals[np.asarray(fl)] = np.nan
+        lib[key][rows] = vals
+        lib['failed'][rows] = fl
+        lib['fail_messages'][rows] = ms
+        lib['done'][rows] = ~np.asarray(fl)
+        if path is not None:
+            save()
+
+    t0 = time.perf_counter()
+    info = {'streams': 0}
+    if todo.size:
+        gwb_for_params(pspace, params[todo], shapes=None if shapes is None else np.asarray(shapes)[todo],
+                       fobs_edges=fobs_edges, nreals=nreals, seeds=seeds[todo], cosmo=cosmo, streams=streams,
+                       chunk=chunk, plan_workers=plan_workers, lookahead=lookahead, log_every=log_every, info=info,
+                       on_block=on_block, collect=False)
+    lib['samples_per_s'] = todo.size / (time.perf_counter() - t0) if todo.size else 0.0
+    lib['streams_used'] = np.array(info['streams'])
+    lib['sample_params'] = np.where(lib['failed'][:, None], np.nan, lib['sample_params'])
+    lib['fail_messages'] = lib['fail_messages'].astype(str)
+    if path is not None and todo.size == 0:
+        save()
     return lib
 
 
@@ -387,7 +509,8 @@ AUTOTUNE_PROBE = 16              # rows timed per candidate
 
 
 def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_REALS, seeds=0, cosmo=None,
-                   streams=3, chunk=256, reduce=None, plan_workers=8, lookahead=16, log_every=0, info=None):
+                   streams=3, chunk=256, reduce=None, plan_workers=8, lookahead=16, log_every=0, info=None,
+                   on_block=None, collect=True):
     """Poisson GWB h_c (n, F, R) for parameter rows ``params`` (n, D) of ``pspace`` (unvaried
     parameters from its ``DEFAULTS``), or ``reduce`` of it per block of samples, concatenated.
 
@@ -404,7 +527,13 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
     ``streams='auto'``: the first rows are processed in blocks of AUTOTUNE_PROBE rows, one block
     per candidate in AUTOTUNE_STREAMS, each timed to completion; the fastest count runs the rest
     (rows are never recomputed: a row's result does not depend on its stream).  ``info``, if a
-    dict, receives the chosen count and the probe rates.
+    dict, receives the chosen count, the probe rates, ``failed`` (S,) and ``fail_messages``.
+
+    A row whose model or launch raises, whose scatter leaves bad values, or whose GWB is not finite
+    fails alone: its h_c is NaN and ``failed`` marks it; the other rows are unaffected.
+    ``on_block(start, hc (n, F, R), failed (n,), messages)`` is called after every block (the
+    checkpoint hook of :func:`gwb_library`); with ``collect=False`` nothing else is kept and None
+    is returned.
     """
     import torch
     from . import engine, _lib, scatter as _scatter
@@ -442,7 +571,11 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
         settings = dict(pspace.DEFAULTS)
         for name, value in zip(pspace.param_names, params[ii]):
             settings[name] = float(value)
-        sam = pspace._init_sam(shape, settings, cosmo=cosmo)
+        try:
+            sam = pspace._init_sam(shape, settings, cosmo=cosmo)
+        except Exception as err:      # reported when the row's turn comes
+            sams_[ii] = (None, err)
+            return
         sams_[ii] = (sam, settings)
         if pool is None or sam._mmbulge._scatter_dex <= 0.0:
             return
@@ -459,9 +592,12 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
     nstream = max(cands) if cands else (3 if auto else max(1, int(streams)))
     streams_ = [torch.cuda.Stream(device=dev) for _ in range(max([nstream] + cands))]
     block = torch.empty((min(max(chunk, AUTOTUNE_PROBE), S), F, R), dtype=torch.float64, pin_memory=True)
-    out = np.zeros((S, F, R)) if reduce is None else []
+    out = (np.zeros((S, F, R)) if reduce is None else []) if collect else None
     inflight = collections.deque()      # one event per launched row: the host stays <= depth rows ahead
     probe_rates = {}
+    failed = np.zeros(S, dtype=bool)
+    messages = [''] * S
+    checks = [None] * block.shape[0]     # per row of a block: the scatter's status tensor
     t0 = time.perf_counter()
     try:
         for ii in range(min(lookahead, S)):
@@ -479,6 +615,12 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
                 ii = start + j
                 prefetch(ii + lookahead)
                 sam, settings = sams_.pop(ii)
+                if sam is None:               # the model itself could not be built
+                    failed[ii] = True
+                    messages[ii] = f"{type(settings).__name__}: {settings}"
+                    block[j].fill_(float("nan"))
+                    checks[j] = None
+                    continue
                 fut = pending.pop(_scatter.plan_key(sam.mtot, sam.mrat), None)
                 if fut is not None:
                     try:
@@ -486,13 +628,21 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
                     except Exception:       # a broken worker pool (e.g. no importable __main__): in line
                         geom, pool = None, None
                     _scatter.put_plan(_scatter.ScatterPlan(sam.mtot, sam.mrat, geom=geom), sam.mtot, sam.mrat)
-                with torch.cuda.stream(streams_[ii % nstream]):
-                    hard = pspace._init_hard(sam, settings)
-                    number, h2fdf, _ = sam._number_h2fdf_device(fobs_edges, hard)
-                    _, _, hc2, _ = engine.realize(number, h2fdf, R, 0, _lib.HD_SEL_NONE, seed=int(seeds[ii]))
-                    block[j].copy_(hc2, non_blocking=True)
-                    ev = torch.cuda.Event()
-                    ev.record()
+                try:
+                    with torch.cuda.stream(streams_[ii % nstream]):
+                        hard = pspace._init_hard(sam, settings)
+                        number, h2fdf, _ = sam._number_h2fdf_device(fobs_edges, hard)
+                        _, _, hc2, _ = engine.realize(number, h2fdf, R, 0, _lib.HD_SEL_NONE, seed=int(seeds[ii]))
+                        block[j].copy_(hc2, non_blocking=True)
+                        ev = torch.cuda.Event()
+                        ev.record()
+                    checks[j] = sam._scatter_iters
+                except Exception as err:   # this row fails, the others go on (gen_lib.py:307-316)
+                    failed[ii] = True
+                    messages[ii] = f"{type(err).__name__}: {err}"
+                    block[j].fill_(float("nan"))
+                    checks[j] = None
+                    continue
                 inflight.append(ev)
                 if len(inflight) > depth:
                     inflight.popleft().synchronize()
@@ -504,7 +654,22 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
                     nstream = max(probe_rates, key=probe_rates.get)
                     cands = []
             hc = np.sqrt(block[:n].numpy())
-            if reduce is None:
+            for j in range(n):   # the scatter's nearest-node fill (sam.py:1377-1383) and finite outputs
+                ii = start + j
+                if failed[ii]:
+                    continue
+                it = checks[j]
+                bad_fill = 0 if it is None else int(it[-1].item())
+                if bad_fill or not np.all(np.isfinite(hc[j])):
+                    failed[ii] = True
+                    messages[ii] = ("ValueError: M-Mbulge scatter left bad values" if bad_fill
+                                    else "FloatingPointError: non-finite GWB")
+                    hc[j] = np.nan
+            if on_block is not None:
+                on_block(start, hc, failed[start:start + n], messages[start:start + n])
+            if not collect:
+                pass
+            elif reduce is None:
                 out[start:start + n] = hc
             else:
                 out.append(np.asarray(reduce(hc)))
@@ -516,7 +681,9 @@ def gwb_for_params(pspace, params, shapes=None, fobs_edges=None, nreals=DEF_NUM_
         for fut in pending.values():
             fut.cancel()
     if isinstance(info, dict):
-        info.update(streams=nstream, probe_rates=probe_rates)
+        info.update(streams=nstream, probe_rates=probe_rates, failed=failed, fail_messages=messages)
+    if not collect:
+        return None
     return out if reduce is None else np.concatenate(out, axis=0)
 
 

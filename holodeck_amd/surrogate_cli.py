@@ -34,12 +34,14 @@ def cmd_generate(a):
     ps = _space(a.nsamples, a.seed, a.shape)
     samples = np.arange(a.part, a.nsamples, a.parts)
     lib = librarian.gwb_library(ps, nfreqs=a.nfreqs, nreals=a.nreals, seed=a.seed, samples=samples,
-                                streams=a.streams, reduce=surrogate.targets_from_gwb, log_every=a.log_every)
-    np.savez(a.out, samples=samples, sample_params=lib['sample_params'],
-             targets=lib['targets'].astype(np.float32), param_names=lib['param_names'],
+                                streams=a.streams, reduce=surrogate.targets_from_gwb, log_every=a.log_every,
+                                path=a.out + ".partial.npz")   # checkpoint: a rerun resumes
+    keep = ~lib['failed']   # failed samples (NaN outputs) are left out of the training set
+    np.savez(a.out, samples=samples[keep], sample_params=lib['sample_params'][keep],
+             targets=lib['targets'][keep].astype(np.float32), param_names=lib['param_names'],
              fobs_cents=lib['fobs_cents'], nreals=np.array(a.nreals), shape=np.array(a.shape),
              extrema=ps.extrema, lhs_seed=np.array(a.seed), lhs_nsamples=np.array(a.nsamples))
-    print(json.dumps({"generated": int(samples.size), "samples_per_s": lib['samples_per_s'],
+    print(json.dumps({"generated": int(keep.sum()), "failed": int((~keep).sum()), "samples_per_s": lib['samples_per_s'],
                       "grid": list(a.shape), "nfreqs": a.nfreqs, "nreals": a.nreals}), flush=True)
 
 

@@ -252,20 +252,25 @@ void hd_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
  * by Philox; exported for tests of the generator. */
 void hd_xoshiro128ss(uint32_t *state, int64_t n, uint32_t *out);
 /* Test hooks of the realization kernels' sampler (no counterpart in the reference, whose
- * numpy Poisson draws they replace): the CDF table of the Poisson(lam) window built by one wave
- * (klo_n = {klo, n}; cdf[n] = floor(CDF(klo + k) 2^64), saturating; cap at most 8000), and
- * inversions of such a table with given words (tab = the n high words then the n low words;
- * two words per draw; slot[i] the window index drawn, used[i] the words it took). */
-int hd_poisson_cdf_table(double lam, int cap, int *klo_n, uint64_t *cdf, void *stream);
-/* The same window on the host (no GPU work): [klo, klo + n) with each dropped tail below 2^-56;
- * n = 0 when it needs more than cap entries or its end would not fit an int. */
-int hd_poisson_window(double lam, int cap, int *klo, int *n);
-int hd_table_draw_words(const uint32_t *tab, int n, const uint32_t *words, int64_t m, int *slot, int *used,
-                        void *stream);
+ * numpy Poisson draws they replace): the CDF table of Poisson(lam) built by one wave
+ * (kb_kt_n = {kb, kt, n}: the support starts at kb, the table holds the counts kt .. kt + n - 1,
+ * cdf[j] = floor(CDF(kt + j) 2^64) with the CDF summed from kb, saturating; cap at most 8000),
+ * and draws from such a table with given words, tail walks included (tab = the n high words then
+ * the n low words, copied to LDS as the kernels hold it; lam, kb, kt those of the table; two
+ * words per draw; count[i] the count drawn, used[i] the words it took). */
+int hd_poisson_cdf_table(double lam, int cap, int *kb_kt_n, uint64_t *cdf, void *stream);
+/* The same table on the host (no GPU work): support start kb (dropping less than 2^-56 below),
+ * table [kt, kt + n) covering lam -/+ 4 sqrt(lam) and the CDF below it; n = 0 when it needs more
+ * than cap entries or the support's end would not fit an int. */
+int hd_poisson_window(double lam, int cap, int *kb, int *kt, int *n);
+int hd_table_draw_words(const uint32_t *tab, int n, double lam, int kb, int kt, const uint32_t *words, int64_t m,
+                        int *count, int *used, void *stream);
 /* The same for two tables drawn together (the kernels' interleaved pair): four words per pair,
- * taken as x_a, x_b, then w_a and w_b where needed; slot[2i], slot[2i+1] the indices drawn. */
-int hd_table_draw_pair_words(const uint32_t *tab, int n, const uint32_t *tab2, int n2, const uint32_t *words,
-                             int64_t m, int *slot, int *used, void *stream);
+ * taken as x_a, x_b, then w_a and w_b where a tie needs them, then where a tail walk needs them;
+ * count[2i], count[2i+1] the counts drawn. */
+int hd_table_draw_pair_words(const uint32_t *tab, int n, double lam, int kb, int kt, const uint32_t *tab2, int n2,
+                             double lam2, int kb2, int kt2, const uint32_t *words, int64_t m, int *count, int *used,
+                             void *stream);
 
 /* ------------------------------------------------------------------------
  * Learned surrogate (MLP: parameters -> log10 h_c percentiles).  No

@@ -105,24 +105,28 @@ def test_density_param_layout_matches_kernel_enum():
 
 
 def test_poisson_window_sizing_on_the_host():
-    """The CDF windows of the realization kernels, computed on the host by the same function: each
-    dropped tail below 2^-56 (scipy's Poisson CDF), at most `cap` entries, and no table (n = 0)
-    when the window is too wide or its end would overflow an int; the hook rejects such a window
+    """The CDF tables of the realization kernels, sized on the host by the same function: the
+    support drops less than 2^-56 below its first count kb (scipy's Poisson CDF), the table covers
+    lam -/+ 4 sqrt(lam) (or reaches the support's ends), at most `cap` entries, and no table (n = 0)
+    when it is too wide or the support's end would overflow an int; the hook rejects such a table
     before launching anything."""
     from scipy.stats import poisson
     from holodeck_amd import _lib
-    cap = 5632
-    for lam in (1e-3, 0.7, 1.0, 6.3, 143.9, 2.5e4, 1.2e5, 2.0e5):
-        klo, n = _lib.poisson_window(lam, cap)
+    cap = 5120
+    for lam in (1e-3, 0.7, 1.0, 6.3, 143.9, 2.5e4, 1.2e5, 2.0e5, 4.0e5):
+        kb, kt, n = _lib.poisson_window(lam, cap)
         if n == 0:
-            assert lam > 1e5      # windows grow like 17.6 sqrt(lam): wider than 5632 slots above ~1e5
+            assert lam > 3.9e5     # tables grow like 8 sqrt(lam): wider than 5120 entries above ~4e5
             continue
-        assert 0 < n <= cap and klo >= 0
-        assert poisson.cdf(klo - 1, lam) < 2.0 ** -56 and poisson.sf(klo + n - 1, lam) < 2.0 ** -56
-    for lam in (2.0 ** 31, 1e12, 1e30, 1e35):   # rounding would collapse these windows to one node
-        assert _lib.poisson_window(lam, cap) == (0, 0)
-    klo, n = _lib.poisson_window(1.0, 8000)
-    assert (klo, n) == (0, 31)
+        sd = np.sqrt(lam)
+        assert 0 < n <= cap and 0 <= kb <= kt
+        assert poisson.cdf(kb - 1, lam) < 2.0 ** -56
+        assert kt == kb or kt + 1 <= lam - 4.0 * sd
+        assert kt + n - 1 >= lam + 4.0 * sd + 1.0 or poisson.sf(kt + n - 1, lam) < 2.0 ** -56
+        assert n <= 8.0 * sd + 5.0 or kt == kb
+    for lam in (2.0 ** 31, 1e12, 1e30, 1e35):   # rounding would collapse these supports to one node
+        assert _lib.poisson_window(lam, cap) == (0, 0, 0)
+    assert _lib.poisson_window(1.0, 8000) == (0, 0, 7)
     with pytest.raises(_lib.HoloHipError, match="exceeds cap"):
         _lib.call("hd_poisson_cdf_table", 1e12, 8000, 1, 1, None)
     with pytest.raises(_lib.HoloHipError, match="bad arguments"):

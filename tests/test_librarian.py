@@ -123,3 +123,99 @@ def test_stream_autotuning_keeps_results(gpu):
     b = L.gwb_for_params(ps, ps.param_samples, fobs_edges=np.arange(1, 6) / 5e8, nreals=8, seeds=3, streams=1)
     assert info["streams"] in L.AUTOTUNE_STREAMS and len(info["probe_rates"]) == len(L.AUTOTUNE_STREAMS)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_run_library_isolates_failures_and_resumes(gpu, tmp_path, monkeypatch):
+    """A sample whose run raises is marked failed (NaN outputs and parameters, a 'fail' file), the
+    others are unaffected; a rerun reads finished samples back and recomputes only failed or missing
+    ones (gen_lib.py:283-320, combine.py:206-212), with the same results."""
+    print("[library failures and resume]", flush=True)
+    ps = L.PS_Classic_Phenom_Uniform(nsamples=4, seed=21)
+    shapes = np.array([[15, 13, 11], [13, 11, 9], [17, 13, 11], [13, 13, 9]])
+    kw = dict(shapes=shapes, nfreqs=5, nreals=12, nloudest=2, seed=4)
+    ref = L.run_library(ps, **kw)
+    assert not ref["failed"].any()
+    real = L.run_model
+    calls = []
+
+    def flaky(sam, hard, **k):
+        calls.append(k["seed"])
+        if k["seed"] == 4 + 2 * 2:            # sample 2
+            raise RuntimeError("planted failure")
+        return real(sam, hard, **k)
+
+    monkeypatch.setattr(L, "run_model", flaky)
+    sims = tmp_path / "sims"
+    lib = L.run_library(ps, sims_dir=str(sims), **kw)
+    assert list(lib["failed"]) == [False, False, True, False]
+    assert "planted failure" in lib["fail_messages"][2]
+    assert np.all(np.isnan(lib["gwb"][2])) and np.all(np.isnan(lib["sample_params"][2]))
+    for i in (0, 1, 3):
+        for key in ("gwb", "hc_ss", "hc_bg"):
+            assert np.array_equal(lib[key][i], ref[key][i]), (i, key)
+    with np.load(L.sim_fname(str(sims), 2)) as fh:
+        assert fh.files == ["fail"]
+    # resume: sample 2 (failed) and sample 1 (file removed) are recomputed, 0 and 3 read back
+    (sims / "sim_000001.npz").unlink()
+    monkeypatch.setattr(L, "run_model", lambda sam, hard, **k: (calls.append(k["seed"]), real(sam, hard, **k))[1])
+    calls.clear()
+    again = L.run_library(ps, sims_dir=str(sims), **kw)
+    assert sorted(calls) == [4 + 2 * 1, 4 + 2 * 2] and int(again["resumed"]) == 2
+    assert not again["failed"].any()
+    for key in ("gwb", "hc_ss", "hc_bg"):
+        assert np.array_equal(again[key], ref[key]), key
+
+
+@pytest.mark.gpu
+def test_gwb_library_isolates_failures_and_resumes(gpu, tmp_path, monkeypatch):
+    """The pipelined library: a row whose model raises fails alone (NaN, flagged); the checkpoint
+    written after every block lets an interrupted run continue with identical results."""
+    print("[gwb_library failures and resume]", flush=True)
+    ps = L.PS_Classic_Phenom_Uniform(nsamples=9, seed=23)
+    shapes = np.array([[13, 11, 9], [15, 11, 9], [13, 13, 11], [11, 11, 9], [13, 11, 11], [15, 13, 9],
+                       [11, 13, 9], [13, 11, 9], [15, 11, 11]])
+    kw = dict(shapes=shapes, nfreqs=4, nreals=10, seed=6, streams=2, chunk=3)
+    ref = L.gwb_library(ps, **kw)
+    assert not ref["failed"].any() and np.all(np.isfinite(ref["gwb"]))
+    real_hard = ps._init_hard
+    bad = ps.param_samples[4]
+
+    def flaky_hard(sam, settings):
+        if all(settings[n] == v for n, v in zip(ps.param_names, bad)):
+            raise ValueError("planted failure")
+        return real_hard(sam, settings)
+
+    monkeypatch.setattr(ps, "_init_hard", flaky_hard)
+    lib = L.gwb_library(ps, **kw)
+    assert list(np.flatnonzero(lib["failed"])) == [4] and "planted failure" in lib["fail_messages"][4]
+    assert np.all(np.isnan(lib["gwb"][4])) and np.all(np.isnan(lib["sample_params"][4]))
+    ok = np.arange(9) != 4
+    assert np.array_equal(lib["gwb"][ok], ref["gwb"][ok])
+    # an interrupted run: the second block raises (outside any row), leaving the first block's checkpoint
+    monkeypatch.setattr(ps, "_init_hard", real_hard)
+    path = str(tmp_path / "lib.npz")
+    real_on = L.gwb_for_params
+
+    def interrupted(*a, **k):
+        on_block = k["on_block"]
+        seen = []
+
+        def once(start, hc, fl, ms):
+            on_block(start, hc, fl, ms)
+            seen.append(start)
+            if len(seen) == 1:
+                raise KeyboardInterrupt
+        k["on_block"] = once
+        return real_on(*a, **k)
+
+    monkeypatch.setattr(L, "gwb_for_params", interrupted)
+    with pytest.raises(KeyboardInterrupt):
+        L.gwb_library(ps, path=path, **kw)
+    with np.load(path, allow_pickle=False) as fh:
+        assert fh["done"].sum() == 3
+    monkeypatch.setattr(L, "gwb_for_params", real_on)
+    done = L.gwb_library(ps, path=path, **kw)
+    assert done["done"].all() and np.array_equal(done["gwb"], ref["gwb"])
+    with pytest.raises(ValueError, match="another library"):
+        L.gwb_library(ps, path=path, **dict(kw, seed=7))
