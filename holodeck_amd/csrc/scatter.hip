@@ -27,16 +27,22 @@
 namespace hd {
 
 // Slot order: slot t is position t of the level schedule (vertex lidx[t]).
-// slot_eoff: exclusive prefix of the slots' degrees (edges of slot t are
-// [slot_eoff[t], slot_eoff[t+1]) in slot-ordered CSR).  One workgroup.
+// Adjacency in slot order, padded to CT_SLOT entries per slot: entry k < CT_SLOT
+// of slot t is at t CT_SLOT + k (neighbour -1 past the degree); the entries of a
+// vertex with more neighbours continue in an overflow area after the N CT_SLOT
+// padded ones, at slot_ovf[t] .. slot_ovf[t+1] (exclusive prefix of the
+// surplus degrees, made here by one workgroup).
+constexpr int CT_SLOT = 8;
+
 __global__ void __launch_bounds__(1024) k_ct_slots(int N, const int32_t* __restrict__ nptr,
-                                                   const int32_t* __restrict__ lidx, int32_t* __restrict__ slot_eoff) {
+                                                   const int32_t* __restrict__ lidx, int32_t* __restrict__ slot_ovf) {
     __shared__ int32_t s_part[1024];
     const int tid = threadIdx.x;
     int32_t carry = 0;
     for (int base = 0; base < N; base += 1024) {
         const int t = base + tid;
-        const int32_t d = (t < N) ? nptr[lidx[t] + 1] - nptr[lidx[t]] : 0;
+        int32_t d = (t < N) ? nptr[lidx[t] + 1] - nptr[lidx[t]] - CT_SLOT : 0;
+        d = d > 0 ? d : 0;
         s_part[tid] = d;
         __syncthreads();
         for (int off = 1; off < 1024; off <<= 1) {
@@ -45,35 +51,41 @@ __global__ void __launch_bounds__(1024) k_ct_slots(int N, const int32_t* __restr
             s_part[tid] += v;
             __syncthreads();
         }
-        if (t < N) slot_eoff[t] = carry + s_part[tid] - d;
+        if (t < N) slot_ovf[t] = carry + s_part[tid] - d;
         carry += s_part[1023];
         __syncthreads();
     }
-    if (tid == 0) slot_eoff[N] = carry;
+    if (tid == 0) slot_ovf[N] = carry;
 }
 
-// Geometry of every slot: the 2x2 system (Q0, Q1, Q3, det) and the edges
-// (neighbour, ex, ey, L^3) in slot-ordered CSR -- the same expressions in the
-// same order as the sweep used to evaluate them, so the sweep's arithmetic is
-// unchanged.  Edge storage holds ecap entries (6 N bounds a planar graph's
-// directed edges); an overflow is reported through *bad.
+// Index of adjacency entry k of slot t in the padded layout.
+__device__ __forceinline__ int64_t ct_entry(int64_t N, int t, int k, const int32_t* __restrict__ slot_ovf) {
+    return (k < CT_SLOT) ? (int64_t)t * CT_SLOT + k : N * CT_SLOT + slot_ovf[t] + (k - CT_SLOT);
+}
+
+// Geometry of every slot: the vertex record {p, overflow begin, overflow end} and
+// {Q0, Q1, Q3, det, 1/det}, the 2x2 system summed over the neighbours in order,
+// and per adjacency entry the neighbour and {ex, ey, L^3, 1/L^3} -- the same
+// expressions in the same order as the sequential sweep evaluates them.  Pads:
+// neighbour -1.  Overflow entries beyond n_ovf are reported through *bad.
 __global__ void __launch_bounds__(256) k_ct_geom(int N, const double* __restrict__ pts, const int32_t* __restrict__ nptr,
                                                  const int32_t* __restrict__ nidx, const int32_t* __restrict__ lidx,
-                                                 const int32_t* __restrict__ slot_eoff, int64_t ecap,
-                                                 double* __restrict__ slot_q, int32_t* __restrict__ e_q,
-                                                 double* __restrict__ e_geo, int32_t* __restrict__ bad) {
+                                                 const int32_t* __restrict__ slot_ovf, int n_ovf,
+                                                 int4* __restrict__ slot_i, double* __restrict__ slot_v,
+                                                 int32_t* __restrict__ pe_q, double4* __restrict__ pe_geo,
+                                                 int32_t* __restrict__ bad) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
+    if (slot_ovf[N] > n_ovf) {
+        if (t == 0) atomicOr(bad, 1 << 20);
+        return;
+    }
     const int p = lidx[t];
     const double px = pts[2 * p], py = pts[2 * p + 1];
     double Q0 = 0, Q1 = 0, Q3 = 0;
-    int64_t e = slot_eoff[t];
-    if ((int64_t)slot_eoff[N] > ecap) {
-        if (t == 0) atomicAdd(bad, 1 << 20);
-        return;
-    }
-    for (int jj = nptr[p]; jj < nptr[p + 1]; jj++, e++) {
-        const int q = nidx[jj];
+    const int deg = nptr[p + 1] - nptr[p];
+    for (int k = 0; k < deg; k++) {
+        const int q = nidx[nptr[p] + k];
         const double ex = pts[2 * q] - px;
         const double ey = pts[2 * q + 1] - py;
         const double L = sqrt(ex * ex + ey * ey);
@@ -81,16 +93,24 @@ __global__ void __launch_bounds__(256) k_ct_geom(int N, const double* __restrict
         Q0 += 4 * ex * ex / L3;
         Q1 += 4 * ex * ey / L3;
         Q3 += 4 * ey * ey / L3;
-        e_q[e] = q;
-        e_geo[3 * e] = ex;
-        e_geo[3 * e + 1] = ey;
-        e_geo[3 * e + 2] = L3;
+        const int64_t e = ct_entry(N, t, k, slot_ovf);
+        pe_q[e] = q;
+        pe_geo[e] = make_double4(ex, ey, L3, 1.0 / L3);
+    }
+    for (int k = deg; k < CT_SLOT; k++) {
+        pe_q[(int64_t)t * CT_SLOT + k] = -1;
+        pe_geo[(int64_t)t * CT_SLOT + k] = make_double4(0.0, 0.0, 1.0, 1.0);
     }
     const double Q2 = Q1;
-    slot_q[4 * t] = Q0;
-    slot_q[4 * t + 1] = Q1;
-    slot_q[4 * t + 2] = Q3;
-    slot_q[4 * t + 3] = Q0 * Q3 - Q1 * Q2;
+    const double det = Q0 * Q3 - Q1 * Q2;
+    slot_i[t] = make_int4(p, slot_ovf[t], slot_ovf[t + 1], 0);
+    double* sv = slot_v + 6 * (int64_t)t;
+    sv[0] = Q0;
+    sv[1] = Q1;
+    sv[2] = Q3;
+    sv[3] = det;
+    sv[4] = 1.0 / det;
+    sv[5] = 0.0;
 }
 
 // dens (N, Z) -> densT (Z, N): each slice contiguous
@@ -101,202 +121,251 @@ __global__ void __launch_bounds__(256) k_transpose_nz(const double* __restrict__
     b[z * N + p] = a[t];
 }
 
-// Fixed-width copy of the slot-ordered geometry: the first CT_D edges of every
-// slot in [d][slot] arrays (padding: neighbour 0, zero edge, L^3 = 1, never read
-// as the slot's degree stops the loop), so a sweep level loads each slot's edges
-// with independent, coalesced reads instead of chasing slot_eoff.
-constexpr int CT_D = 8;
-__global__ void __launch_bounds__(256) k_ct_pad(int N, const int32_t* __restrict__ slot_eoff,
-                                                const int32_t* __restrict__ e_q, const double* __restrict__ e_geo,
-                                                int32_t* __restrict__ s_deg, int32_t* __restrict__ pq,
-                                                double* __restrict__ pex, double* __restrict__ pey,
-                                                double* __restrict__ pl3) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N) return;
-    const int e0 = slot_eoff[t], deg = slot_eoff[t + 1] - e0;
-    s_deg[t] = deg;
-#pragma unroll
-    for (int d = 0; d < CT_D; d++) {
-        const bool in = d < deg;
-        pq[d * N + t] = in ? e_q[e0 + d] : 0;
-        pex[d * N + t] = in ? e_geo[3 * (e0 + d)] : 0.0;
-        pey[d * N + t] = in ? e_geo[3 * (e0 + d) + 1] : 0.0;
-        pl3[d * N + t] = in ? e_geo[3 * (e0 + d) + 2] : 1.0;
-    }
-}
-
-// The per-slice constant of every padded edge, 6 (f1 - f2) (the sweep's first
-// term, the same operation on the same values), A[z][d][slot].
-__global__ void __launch_bounds__(256) k_ct_fdiff(const double* __restrict__ densT, int N, int Z,
-                                                  const int32_t* __restrict__ lidx, const int32_t* __restrict__ s_deg,
-                                                  const int32_t* __restrict__ pq, double* __restrict__ A) {
+// The per-slice constant of every adjacency entry, 6 (f1 - f2) (the sweep's first
+// term, the same operation on the same values), A[z][entry]; 0 for pads.
+__global__ void __launch_bounds__(256) k_ct_fdiff(const double* __restrict__ densT, int N, int Z, int64_t ecap,
+                                                  const int4* __restrict__ slot_i, const int32_t* __restrict__ pe_q,
+                                                  double* __restrict__ A) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)N * Z) return;
     const int z = (int)(i / N), t = (int)(i % N);
     const double* frow = densT + (int64_t)z * N;
-    const double f1 = frow[lidx[t]];
-    const int deg = s_deg[t];
-#pragma unroll
-    for (int d = 0; d < CT_D; d++)
-        A[((int64_t)z * CT_D + d) * N + t] = (d < deg) ? 6 * (f1 - frow[pq[d * N + t]]) : 0.0;
+    const int4 si = slot_i[t];
+    const double f1 = frow[si.x];
+    double* Az = A + (int64_t)z * ecap;
+    for (int k = 0; k < CT_SLOT; k++) {
+        const int64_t e = (int64_t)t * CT_SLOT + k;
+        const int q = pe_q[e];
+        Az[e] = (q >= 0) ? 6 * (f1 - frow[q]) : 0.0;
+    }
+    for (int64_t e = (int64_t)N * CT_SLOT + si.y; e < (int64_t)N * CT_SLOT + si.z; e++)
+        Az[e] = 6 * (f1 - frow[pe_q[e]]);
 }
 
-// One slot's inputs to a sweep step, held in registers (loaded one level ahead).
-struct CtSlot {
-    int p, deg, e0;
-    double Q0, Q1, Q3, det, f1;
-    int q[CT_D];
-    double ex[CT_D], ey[CT_D], l3[CT_D], a[CT_D];
-};
-
-__device__ __forceinline__ void ct_load(CtSlot& s, int t, bool valid, int N, const double* __restrict__ frow,
-                                        const double* __restrict__ Az, const int32_t* __restrict__ lidx,
-                                        const int32_t* __restrict__ s_deg, const int32_t* __restrict__ slot_eoff,
-                                        const double* __restrict__ slot_q, const int32_t* __restrict__ pq,
-                                        const double* __restrict__ pex, const double* __restrict__ pey,
-                                        const double* __restrict__ pl3) {
-    s.deg = 0;
-    if (!valid) return;
-    s.p = lidx[t];
-    s.deg = s_deg[t];
-    s.e0 = slot_eoff[t];
-    s.Q0 = slot_q[4 * t];
-    s.Q1 = slot_q[4 * t + 1];
-    s.Q3 = slot_q[4 * t + 2];
-    s.det = slot_q[4 * t + 3];
-    s.f1 = frow[s.p];
-#pragma unroll
-    for (int d = 0; d < CT_D; d++) {
-        s.q[d] = pq[d * N + t];
-        s.ex[d] = pex[d * N + t];
-        s.ey[d] = pey[d * N + t];
-        s.l3[d] = pl3[d * N + t];
-        s.a[d] = Az[(int64_t)d * N + t];
-    }
+// Markstein's final division step (P. Markstein, IBM J. Res. Dev. 34 (1990) 111):
+// for b > 0 and rb = RN(1/b), q0 = RN(a rb), the remainder a - b q0 exactly by one
+// fma, q = RN(q0 + remainder rb) is a / b correctly rounded while |q0| stays in
+// [2^-900, 2^900] (so neither the remainder nor the correction leaves the normal
+// range), and for a zero a it gives q0, the zero a / b gives.  ct_div_ok tells
+// whether one of the two holds.
+__device__ __forceinline__ double ct_div_fast(double a, double b, double rb, double q0) {
+    const double r = fma(b, q0, -a);   // -(a - b q0): +0 when exact, so a zero a keeps its sign in q0
+    return fma(-r, rb, q0);
+}
+__device__ __forceinline__ bool ct_div_ok(double a, double q0) {
+    const double aq = fabs(q0);
+    return ((aq >= 0x1p-900) & (aq <= 0x1p+900)) | (a == 0.0);   // no short circuit: no branches
 }
 
-// One Gauss-Seidel update of slot s: scipy's arithmetic, term by term and in the
-// same order (the first CT_D edges from registers, the rest of a wider slot from
-// the slot-ordered CSR).  Returns the relative change.
-__device__ __forceinline__ double ct_step(const CtSlot& c, double* y, const double* __restrict__ frow,
-                                          const int32_t* __restrict__ e_q, const double* __restrict__ e_geo) {
-    double s0 = 0, s1 = 0;
-#pragma unroll
-    for (int d = 0; d < CT_D; d++) {
-        if (d < c.deg) {
-            const int q = c.q[d];
-            const double ex = c.ex[d], ey = c.ey[d], L3 = c.l3[d];
-            const double df2 = -ex * y[2 * q] - ey * y[2 * q + 1];
-            s0 += (c.a[d] - 2 * df2) * ex / L3;
-            s1 += (c.a[d] - 2 * df2) * ey / L3;
-        }
+// scipy's two terms of one edge, ((6 (f1 - f2) - 2 df2) ex) / L^3 and
+// ((6 (f1 - f2) - 2 df2) ey) / L^3 with df2 = -ex y[q].x - ey y[q].y; exactly
+// rounded divisions (the IEEE division itself where the fast step's range test
+// fails).
+__device__ __forceinline__ double2 ct_terms(double a, const double4 g, const double2 yq) {
+    const double df2 = -g.x * yq.x - g.y * yq.y;
+    const double u = a - 2 * df2;
+    const double n0 = u * g.x, n1 = u * g.y;
+    const double q0 = n0 * g.w, q1 = n1 * g.w;
+    double2 r;
+    r.x = ct_div_fast(n0, g.z, g.w, q0);
+    r.y = ct_div_fast(n1, g.z, g.w, q1);
+    if (!(ct_div_ok(n0, q0) & ct_div_ok(n1, q1))) {   // rare: exactly rounded either way
+        r.x = n0 / g.z;
+        r.y = n1 / g.z;
     }
-    for (int e = c.e0 + CT_D; e < c.e0 + c.deg; e++) {
-        const int q = e_q[e];
-        const double ex = e_geo[3 * e], ey = e_geo[3 * e + 1], L3 = e_geo[3 * e + 2];
-        const double f2 = frow[q];
-        const double df2 = -ex * y[2 * q] - ey * y[2 * q + 1];
-        s0 += (6 * (c.f1 - f2) - 2 * df2) * ex / L3;
-        s1 += (6 * (c.f1 - f2) - 2 * df2) * ey / L3;
-    }
-    const double Q0 = c.Q0, Q1 = c.Q1, Q3 = c.Q3, det = c.det;
-    const double Q2 = Q1;
-    const double r0 = (Q3 * s0 - Q1 * s1) / det;
-    const double r1 = (-Q2 * s0 + Q0 * s1) / det;
-    const int p = c.p;
-    double a0 = fabs(y[2 * p] + r0), a1 = fabs(y[2 * p + 1] + r1);
-    double change = (a0 > a1) ? a0 : a1;
-    y[2 * p] = -r0;
-    y[2 * p + 1] = -r1;
-    double m0 = fabs(r0), m1 = fabs(r1);
-    double mr = (m0 > m1) ? m0 : m1;
-    change /= (1.0 > mr) ? 1.0 : mr;
-    return change;
+    return r;
 }
 
-// Gauss-Seidel sweeps of one slice (one wave).  Each level's slot inputs are
-// loaded while the previous level is computed (they do not depend on the
-// solution), into the other of two register buffers, so a level costs its LDS
-// reads of the neighbours' gradients and its arithmetic.  With the solution in
-// LDS the single wave's LDS operations complete in issue order, so a level's
-// reads see the previous level's writes without a barrier.
+// LDS writes complete, then the workgroup barrier; the loads in flight are not
+// waited for (the inline asm is also a compiler memory barrier, so no LDS access
+// moves across it).
+__device__ __forceinline__ void ct_barrier_lds() {
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Gauss-Seidel sweeps of one slice by one workgroup.  Within a level, a group of
+// CT_SLOT lanes per vertex: lane k computes the terms of adjacency entry k (the
+// inputs loaded one level ahead) into LDS, -0.0 for a pad (x + -0.0 = x exactly,
+// so a pad changes no sum); the group's first lane reads them back (the lanes
+// are in its own wavefront, whose LDS operations complete in order, so no
+// barrier), adds them in the neighbour order as the sequential loop does, adds
+// the terms of any overflow entries itself, solves the 2x2 system and writes the
+// vertex's gradient.  One barrier per level.  Arithmetic and order are scipy's
+// term by term, so the gradients equal a sequential sweep's bit for bit.  The
+// convergence measure, max over vertices of max|y_old + r| / max(1, max|r|) with
+// y_new = -r, is taken at the end of each sweep from the previous sweep's
+// gradients kept in grad_g (the same operations on the same values).  The
+// workgroup holds CT_SLOT lanes for every vertex of the widest level; the
+// kernel checks that and refuses otherwise.
+constexpr int CT_MAX_THREADS = 1024;
+constexpr int CT_SWEEP_UNROLL = 4;   // vertices per thread in flight in the end-of-sweep pass
+
 template <bool LDS_Y>
-__global__ void __launch_bounds__(64) k_ct_gradients(const double* __restrict__ densT, int N, int Z,
-                                                     const int32_t* __restrict__ lidx,
-                                                     const int32_t* __restrict__ slot_eoff,
-                                                     const double* __restrict__ slot_q, const int32_t* __restrict__ e_q,
-                                                     const double* __restrict__ e_geo, const int32_t* __restrict__ s_deg,
-                                                     const int32_t* __restrict__ pq, const double* __restrict__ pex,
-                                                     const double* __restrict__ pey, const double* __restrict__ pl3,
-                                                     const double* __restrict__ A, const int32_t* __restrict__ lptr,
-                                                     int n_levels, double tol, int maxiter, double* __restrict__ grad_g,
-                                                     int32_t* __restrict__ iters_out) {
-    extern __shared__ double s_y[];
+__global__ void __launch_bounds__(CT_MAX_THREADS) k_ct_gradients(
+    int N, int Z, const int4* __restrict__ slot_i, const double* __restrict__ slot_v, const int32_t* __restrict__ pe_q,
+    const double4* __restrict__ pe_geo, const double* __restrict__ A, int64_t ecap, const int32_t* __restrict__ lptr,
+    int n_levels, double tol, int maxiter, double* __restrict__ ybuf, double* __restrict__ grad_g,
+    int32_t* __restrict__ iters_out) {
+    extern __shared__ double4 s_dyn[];
+    __shared__ double s_err[CT_MAX_THREADS / 64];
+    __shared__ int s_bad;
     const int z = blockIdx.x;
-    const int lane = threadIdx.x;
-    const double* frow = densT + (int64_t)z * N;
-    const double* Az = A + (int64_t)z * CT_D * N;
-    double* y = LDS_Y ? s_y : (grad_g + (int64_t)z * N * 2);
-    // the level pointers next to the solution (or alone) in LDS
-    int32_t* s_lptr = (int32_t*)(LDS_Y ? (s_y + 2 * (size_t)N) : s_y);
-    for (int i = lane; i < 2 * N; i += 64) y[i] = 0.0;
-    for (int i = lane; i <= n_levels; i += 64) s_lptr[i] = lptr[i];
+    const int nthr = blockDim.x, nw = nthr >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int j = tid / CT_SLOT, k = tid % CT_SLOT;
+    const double* Az = A + (int64_t)z * ecap;
+    double2* gz = (double2*)(grad_g + (int64_t)z * N * 2);
+    // LDS: [solution N double2 (if LDS_Y)] [terms nthr double2] [level slot pointers n+1]
+    double2* y = LDS_Y ? (double2*)s_dyn : (double2*)(ybuf + (int64_t)z * N * 2);
+    double2* T = LDS_Y ? (double2*)s_dyn + N : (double2*)s_dyn;
+    int32_t* s_lptr = (int32_t*)(T + nthr);
+    for (int i = tid; i < N; i += nthr) {
+        y[i] = make_double2(0.0, 0.0);
+        gz[i] = make_double2(0.0, 0.0);   // the previous sweep's gradients (read back by this thread only)
+    }
+    if (tid == 0) s_bad = 0;
+    for (int i = tid; i <= n_levels; i += nthr) s_lptr[i] = lptr[i];
     __syncthreads();
-    auto load = [&](CtSlot& b, int lev) {
-        const int t = s_lptr[lev] + lane;
-        ct_load(b, t, t < s_lptr[lev + 1], N, frow, Az, lidx, s_deg, slot_eoff, slot_q, pq, pex, pey, pl3);
-    };
-    auto level_done = [&]() {
-        if (LDS_Y) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = tid; i < n_levels; i += nthr)   // a level wider than the workgroup: refuse
+        if ((s_lptr[i + 1] - s_lptr[i]) * CT_SLOT > nthr) s_bad = 1;
+    __syncthreads();
+    if (s_bad) {
+        if (tid == 0) {
+            atomicOr(iters_out + Z, 1 << 21);
+            iters_out[z] = 0;
+        }
+        return;
+    }
+    const uint32_t elast = (uint32_t)N * CT_SLOT - 1;
+    // this lane's adjacency entry and vertex, one level ahead; every load issued on
+    // every lane (indices clamped into the arrays), so no load sits under a branch
+    // and a use waits for its own loads only.  Byte offsets are 32-bit (the host
+    // checks the arrays' sizes), so the loads take a scalar base and a 32-bit offset.
+    int cq;
+    double4 cg;
+    double ca;
+    int4 vi;
+    double vQ0, vQ1, vQ3, vdet, vrdet;
+    auto load_level = [&](int l0, bool vert) {
+        if (!vert) {
+            const uint32_t e = min((uint32_t)((l0 + j) * CT_SLOT + k), elast);
+            cq = *(const int32_t*)((const char*)pe_q + (e << 2));
+            cg = *(const double4*)((const char*)pe_geo + (e << 5));
+            ca = *(const double*)((const char*)Az + (e << 3));
         } else {
-            __threadfence_block();
+            const uint32_t t = min((uint32_t)(l0 + j), (uint32_t)(N - 1));
+            vi = *(const int4*)((const char*)slot_i + (t << 4));
+            const double* sv = (const double*)((const char*)slot_v + t * 48u);
+            vQ0 = sv[0];
+            vQ1 = sv[1];
+            vQ3 = sv[2];
+            vdet = sv[3];
+            vrdet = sv[4];
         }
     };
-    CtSlot b0, b1;
-    load(b0, 0);
-    int iters = 0, it = 0, lev = 0;
-    double err = 0.0;
-    // the steps (sweep, level) in order, two per trip: b0 holds the even steps, b1 the odd
-    // ones; returns true when a sweep ends converged or at maxiter
-    auto finish_level = [&]() -> bool {
-        level_done();
-        if (++lev < n_levels) return false;
+    int lb0 = s_lptr[0], lb1 = s_lptr[1];
+    int nb0 = s_lptr[n_levels > 1 ? 1 : 0], nb1 = s_lptr[n_levels > 1 ? 2 : 1];
+    load_level(lb0, false);
+    load_level(lb0, true);
+    int lev = 0, it = 0, iters = 0;
+    for (;;) {
+        const bool act = j < lb1 - lb0;
+        // the entry's terms
+        if (act) {
+            const double2 yq = y[cq >= 0 ? cq : 0];
+            double2 tt = ct_terms(ca, cg, yq);
+            if (cq < 0) tt = make_double2(-0.0, -0.0);
+            T[tid] = tt;
+        }
+        load_level(nb0, false);   // the next level's entries
+        asm volatile("" ::: "memory");
+        // the group's first lane: in-order sums, overflow entries, solve
+        if (act && k == 0) {
+            double2 tk[CT_SLOT];
+#pragma unroll
+            for (int i = 0; i < CT_SLOT; i++) tk[i] = T[tid + i];
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int i = 0; i < CT_SLOT; i++) {
+                s0 += tk[i].x;
+                s1 += tk[i].y;
+            }
+            for (int64_t e = (int64_t)N * CT_SLOT + vi.y; e < (int64_t)N * CT_SLOT + vi.z; e++) {   // rare
+                const int q = pe_q[e];
+                const double2 tt = ct_terms(Az[e], pe_geo[e], y[q]);
+                s0 += tt.x;
+                s1 += tt.y;
+            }
+            const double Q2 = vQ1;
+            const double n0 = vQ3 * s0 - vQ1 * s1;
+            const double n1 = -Q2 * s0 + vQ0 * s1;
+            const double q0 = n0 * vrdet, q1 = n1 * vrdet;
+            double r0 = ct_div_fast(n0, vdet, vrdet, q0);
+            double r1 = ct_div_fast(n1, vdet, vrdet, q1);
+            if (!((vdet > 0.0) & ct_div_ok(n0, q0) & ct_div_ok(n1, q1))) {   // rare: exactly rounded either way
+                r0 = n0 / vdet;
+                r1 = n1 / vdet;
+            }
+            y[vi.x] = make_double2(-r0, -r1);
+        }
+        load_level(nb0, true);   // the next level's vertices
+        {
+            const int ln2 = (lev + 2 < n_levels) ? lev + 2 : lev + 2 - n_levels;   // wraps into the next sweep
+            lb0 = nb0;
+            lb1 = nb1;
+            nb0 = s_lptr[ln2 < n_levels ? ln2 : 0];
+            nb1 = s_lptr[(ln2 < n_levels ? ln2 : 0) + 1];
+        }
+        if (LDS_Y) ct_barrier_lds();
+        else __syncthreads();
+        if (++lev < n_levels) continue;
+        // end of a sweep: the largest relative change over every vertex, against the
+        // previous sweep's gradients, which the new ones then replace
         lev = 0;
+        it++;
+        double err = 0.0;
+        for (int base = 0; base < N; base += CT_SWEEP_UNROLL * nthr) {
+            double2 yn[CT_SWEEP_UNROLL], yo[CT_SWEEP_UNROLL];
+#pragma unroll
+            for (int u = 0; u < CT_SWEEP_UNROLL; u++) {   // loads in flight together (index clamped)
+                const int i = min(base + u * nthr + tid, N - 1);
+                yn[u] = y[i];
+                yo[u] = gz[i];
+            }
+#pragma unroll
+            for (int u = 0; u < CT_SWEEP_UNROLL; u++) {
+                const int i = base + u * nthr + tid;
+                if (i < N) {
+                    const double a0 = fabs(yo[u].x - yn[u].x), a1 = fabs(yo[u].y - yn[u].y);   // |y_old + r|, r = -y_new
+                    const double m0 = fabs(yn[u].x), m1 = fabs(yn[u].y);
+                    const double num = (a0 > a1) ? a0 : a1;
+                    const double mr = (m0 > m1) ? m0 : m1;
+                    const double den = (1.0 > mr) ? 1.0 : mr;
+                    const double ch = num / den;
+                    err = (err > ch) ? err : ch;
+                    gz[i] = yn[u];
+                }
+            }
+        }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             double o = __shfl_xor(err, off, 64);
             err = (err > o) ? err : o;
         }
-        it++;
-        if (err < tol) {
-            iters = it;
-            return true;
-        }
-        err = 0.0;
-        return it >= maxiter;
-    };
-    for (;;) {
-        load(b1, (lev + 1 < n_levels) ? lev + 1 : 0);
-        if (b0.deg > 0) {
-            const double ch = ct_step(b0, y, frow, e_q, e_geo);
-            err = (err > ch) ? err : ch;
-        }
-        if (finish_level()) break;
-        load(b0, (lev + 1 < n_levels) ? lev + 1 : 0);
-        if (b1.deg > 0) {
-            const double ch = ct_step(b1, y, frow, e_q, e_geo);
-            err = (err > ch) ? err : ch;
-        }
-        if (finish_level()) break;
-    }
-    if (LDS_Y) {
+        if (lane == 0) s_err[w] = err;
         __syncthreads();
-        for (int i = lane; i < 2 * N; i += 64) grad_g[(int64_t)z * N * 2 + i] = y[i];
+        double m = s_err[0];
+        for (int i = 1; i < nw; i++) m = (m > s_err[i]) ? m : s_err[i];
+        __syncthreads();   // s_err is read by every wave before the next sweep can write it
+        if (m < tol) {
+            iters = it;
+            break;
+        }
+        if (it >= maxiter) break;
     }
-    if (lane == 0) iters_out[z] = iters;
+    if (tid == 0) iters_out[z] = iters;
 }
 
 // Clough-Tocher value at barycentric b in triangle s (vertex order of `simp`).
@@ -475,87 +544,85 @@ extern "C" int hd_scatter_weights_T(const double* dm, int G, double* wT, void* s
     return check_launch("k_rolled_weights_T");
 }
 
-extern "C" size_t hd_scatter_workspace_size(int M, int Q, int Z, int G) {
+extern "C" size_t hd_scatter_workspace_size(int M, int Q, int Z, int G, int n_ovf) {
     size_t N = (size_t)M * Q;
-    const size_t ecap = 6 * N;
+    const size_t ecap = CT_SLOT * N + (size_t)(n_ovf > 0 ? n_ovf : 0);
     return align256(N * Z * 2 * 8) + 2 * align256((size_t)G * G * Z * 8) + align256(N * Z * 8) +   // grad, D, Y, densT
-           align256((N + 1) * 4) + align256(N * 4 * 8) + align256(ecap * 4) + align256(ecap * 3 * 8) +
-           align256(N * 4) + align256(CT_D * N * 4) + 3 * align256(CT_D * N * 8) +   // padded geometry
-           align256((size_t)Z * CT_D * N * 8) + 256;                              // 6 (f1 - f2) per slice
+           align256((N + 1) * 4) + align256(N * 16) + align256(N * 6 * 8) +   // slot overflow offsets and records
+           align256(ecap * 4) + align256(ecap * 32) +                         // adjacency entries
+           align256(N * Z * 2 * 8) +                                          // solution when it exceeds the LDS
+           align256((size_t)Z * ecap * 8) + 256;                              // 6 (f1 - f2) per slice and entry
 }
 
 extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double* pts, const int32_t* nbr_ptr,
                           const int32_t* nbr_idx, const int32_t* levels_ptr, const int32_t* levels_idx, int n_levels,
+                          int max_level_width, int n_ovf,
                           const int32_t* simplices, const double* simplex_g, const int32_t* q_simplex,
                           const double* q_bary, const int32_t* q_nearest, const double* P, const int32_t* back_idx,
                           const double* back_w, double tol, int maxiter, void* workspace, size_t ws_bytes,
                           int32_t* iters_out, void* stream) {
-    if (M < 2 || Q < 2 || Z < 1 || G < 2 || n_levels < 1 || !dens || !pts || !nbr_ptr || !nbr_idx || !levels_ptr ||
-        !levels_idx || !simplices || !simplex_g || !q_simplex || !q_bary || !q_nearest || !P || !back_idx || !back_w ||
-        !iters_out)
+    if (M < 2 || Q < 2 || Z < 1 || G < 2 || n_levels < 1 || max_level_width < 1 || n_ovf < 0 || !dens || !pts ||
+        !nbr_ptr || !nbr_idx || !levels_ptr || !levels_idx || !simplices || !simplex_g || !q_simplex || !q_bary ||
+        !q_nearest || !P || !back_idx || !back_w || !iters_out)
         return set_error(HD_ERR_ARG, "hd_scatter: bad arguments");
     if ((int64_t)G * G > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_scatter: G too large");
-    size_t need = hd_scatter_workspace_size(M, Q, Z, G);
+    size_t need = hd_scatter_workspace_size(M, Q, Z, G, n_ovf);
     if (!workspace || ws_bytes < need) return set_error(HD_ERR_SIZE, "hd_scatter: workspace too small");
     hipStream_t st = (hipStream_t)stream;
     const int N = M * Q;
+    const int64_t ecap = (int64_t)CT_SLOT * N + n_ovf;
     char* p = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
     double* grad = (double*)p;  p += align256((size_t)N * Z * 2 * 8);
     double* Dm = (double*)p;    p += align256((size_t)G * G * Z * 8);
     double* Y = (double*)p;     p += align256((size_t)G * G * Z * 8);
     double* densT = (double*)p; p += align256((size_t)N * Z * 8);
-    const int64_t ecap = 6 * (int64_t)N;
-    int32_t* slot_eoff = (int32_t*)p;  p += align256((size_t)(N + 1) * 4);
-    double* slot_q = (double*)p;       p += align256((size_t)N * 4 * 8);
-    int32_t* e_q = (int32_t*)p;        p += align256((size_t)ecap * 4);
-    double* e_geo = (double*)p;        p += align256((size_t)ecap * 3 * 8);
-    int32_t* s_deg = (int32_t*)p;      p += align256((size_t)N * 4);
-    int32_t* pq = (int32_t*)p;         p += align256((size_t)CT_D * N * 4);
-    double* pex = (double*)p;          p += align256((size_t)CT_D * N * 8);
-    double* pey = (double*)p;          p += align256((size_t)CT_D * N * 8);
-    double* pl3 = (double*)p;          p += align256((size_t)CT_D * N * 8);
+    int32_t* slot_ovf = (int32_t*)p;  p += align256((size_t)(N + 1) * 4);
+    int4* slot_i = (int4*)p;          p += align256((size_t)N * 16);
+    double* slot_v = (double*)p;      p += align256((size_t)N * 6 * 8);
+    int32_t* pe_q = (int32_t*)p;      p += align256((size_t)ecap * 4);
+    double4* pe_geo = (double4*)p;    p += align256((size_t)ecap * 32);
+    double* ybuf = (double*)p;        p += align256((size_t)N * Z * 2 * 8);
     double* fdiff = (double*)p;
     int32_t* bad = iters_out + Z;   // iters_out has Z + 1 entries
 
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, bad, 1);
-    hipLaunchKernelGGL(k_ct_slots, dim3(1), dim3(1024), 0, st, N, nbr_ptr, levels_idx, slot_eoff);
+    hipLaunchKernelGGL(k_ct_slots, dim3(1), dim3(1024), 0, st, N, nbr_ptr, levels_idx, slot_ovf);
     hipLaunchKernelGGL(k_ct_geom, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, st, N, pts, nbr_ptr, nbr_idx, levels_idx,
-                       slot_eoff, ecap, slot_q, e_q, e_geo, bad);
+                       slot_ovf, n_ovf, slot_i, slot_v, pe_q, pe_geo, bad);
     hipLaunchKernelGGL(k_transpose_nz, dim3((unsigned)cdiv((int64_t)N * Z, 256)), dim3(256), 0, st, dens, N, Z, densT);
-    hipLaunchKernelGGL(k_ct_pad, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, st, N, slot_eoff, e_q, e_geo, s_deg, pq,
-                       pex, pey, pl3);
-    hipLaunchKernelGGL(k_ct_fdiff, dim3((unsigned)cdiv((int64_t)N * Z, 256)), dim3(256), 0, st, densT, N, Z,
-                       levels_idx, s_deg, pq, fdiff);
+    hipLaunchKernelGGL(k_ct_fdiff, dim3((unsigned)cdiv((int64_t)N * Z, 256)), dim3(256), 0, st, densT, N, Z, ecap,
+                       slot_i, pe_q, fdiff);
     int rc0 = check_launch("k_ct_fdiff");
     if (rc0) return rc0;
-    // solution vector (and the level pointers) in LDS when they fit
+    // solution vector (and the level pointers and entry terms) in LDS when they fit
     int dev = 0;
     hipError_t he = hipGetDevice(&dev);
     int max_lds = 0;
     if (he == hipSuccess) he = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (he != hipSuccess) return set_error(HD_ERR_HIP, "hd_scatter: device query: %s", hipGetErrorString(he));
-    const size_t lptr_bytes = (size_t)(n_levels + 1) * sizeof(int32_t);
-    const size_t y_bytes = (size_t)N * 2 * sizeof(double) + lptr_bytes;
-    if (lptr_bytes > (size_t)max_lds) return set_error(HD_ERR_SIZE, "hd_scatter: %d levels exceed the LDS", n_levels);
-    if (y_bytes <= (size_t)max_lds) {
-        if (y_bytes > 65536) {
-            he = hipFuncSetAttribute((const void*)k_ct_gradients<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)y_bytes);
-            if (he != hipSuccess) return set_error(HD_ERR_HIP, "hd_scatter: LDS attribute: %s", hipGetErrorString(he));
-        }
-        hipLaunchKernelGGL(k_ct_gradients<true>, dim3(Z), dim3(64), y_bytes, st, densT, N, Z, levels_idx, slot_eoff,
-                           slot_q, e_q, e_geo, s_deg, pq, pex, pey, pl3, fdiff, levels_ptr, n_levels, tol, maxiter,
-                           grad, iters_out);
-    } else {
-        if (lptr_bytes > 65536) {
-            he = hipFuncSetAttribute((const void*)k_ct_gradients<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lptr_bytes);
-            if (he != hipSuccess) return set_error(HD_ERR_HIP, "hd_scatter: LDS attribute: %s", hipGetErrorString(he));
-        }
-        hipLaunchKernelGGL(k_ct_gradients<false>, dim3(Z), dim3(64), lptr_bytes, st, densT, N, Z, levels_idx, slot_eoff,
-                           slot_q, e_q, e_geo, s_deg, pq, pex, pey, pl3, fdiff, levels_ptr, n_levels, tol, maxiter,
-                           grad, iters_out);
+    // CT_SLOT lanes per vertex of the widest level (the kernel refuses a level wider than its workgroup)
+    const int nthr = (int)(((int64_t)max_level_width * CT_SLOT + 63) / 64 * 64);
+    if ((uint64_t)ecap * 32 >= (1ull << 32))   // the sweep addresses its arrays with 32-bit byte offsets
+        return set_error(HD_ERR_SIZE, "hd_scatter: %d nodes exceed the sweep's 32-bit offsets", N);
+    if (nthr > CT_MAX_THREADS)
+        return set_error(HD_ERR_SIZE, "hd_scatter: a level of %d vertices exceeds %d threads", max_level_width,
+                         CT_MAX_THREADS);
+    const size_t aux_bytes = (size_t)nthr * 2 * sizeof(double) + (size_t)(n_levels + 1) * sizeof(int32_t);
+    const size_t y_bytes = (size_t)N * 2 * sizeof(double) + aux_bytes;
+    if (aux_bytes > (size_t)max_lds) return set_error(HD_ERR_SIZE, "hd_scatter: %d levels exceed the LDS", n_levels);
+    const bool lds_y = y_bytes <= (size_t)max_lds;
+    const void* kfn = lds_y ? (const void*)k_ct_gradients<true> : (const void*)k_ct_gradients<false>;
+    const size_t lds_bytes = lds_y ? y_bytes : aux_bytes;
+    if (lds_bytes > 65536) {
+        he = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (he != hipSuccess) return set_error(HD_ERR_HIP, "hd_scatter: LDS attribute: %s", hipGetErrorString(he));
     }
+    if (lds_y)
+        hipLaunchKernelGGL(k_ct_gradients<true>, dim3(Z), dim3(nthr), lds_bytes, st, N, Z, slot_i, slot_v, pe_q,
+                           pe_geo, fdiff, ecap, levels_ptr, n_levels, tol, maxiter, ybuf, grad, iters_out);
+    else
+        hipLaunchKernelGGL(k_ct_gradients<false>, dim3(Z), dim3(nthr), lds_bytes, st, N, Z, slot_i, slot_v, pe_q,
+                           pe_geo, fdiff, ecap, levels_ptr, n_levels, tol, maxiter, ybuf, grad, iters_out);
     int rc = check_launch("k_ct_gradients");
     if (rc) return rc;
     dim3 g1((unsigned)cdiv((int64_t)G * G, 256), Z);

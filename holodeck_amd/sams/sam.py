@@ -241,8 +241,12 @@ class Semi_Analytic_Model:
         it = self._scatter_iters
         if it is not None:
             bad = int(it[-1].item())
+            if bad & (1 << 21):
+                raise ValueError("M-Mbulge scatter: a level of the sweep schedule has more vertices than the "
+                                 "plan's max_level_width")
             if bad >= (1 << 20):
-                raise ValueError("M-Mbulge scatter: the triangulation has more than 6 N directed edges")
+                raise ValueError("M-Mbulge scatter: the triangulation has more overflow adjacency entries than "
+                                 "the plan's n_ovf")
             if bad:
                 raise ValueError(f"After 0th order interpolation, {bad} bad values remain!")
 

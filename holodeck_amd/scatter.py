@@ -136,6 +136,10 @@ def build_geometry(mtot, mrat, refine=4, kd_workers=-1):
         lo_nb = nb[nb < p]
         level[p] = (level[lo_nb].max() + 1) if lo_nb.size else 0
     nlev = int(level.max()) + 1
+    # the sweep runs eight lanes per vertex of a level; entries past a vertex's eighth go to an
+    # overflow area
+    max_level_width = int(np.bincount(level, minlength=nlev).max())
+    n_ovf = int(np.maximum(np.diff(indptr) - 8, 0).sum())
     lorder = np.argsort(level, kind='stable')
     lptr = np.concatenate([[0], np.cumsum(np.bincount(level, minlength=nlev))])
     # regular-grid query points xi[i*G + j] = (mgrid[i], mgrid[j])
@@ -163,7 +167,8 @@ def build_geometry(mtot, mrat, refine=4, kd_workers=-1):
         simplices=tri.simplices.astype(np.int32), simplex_g=_simplex_g(tri), q_simplex=simp.astype(np.int32),
         q_bary=bary, q_nearest=nearest.astype(np.int32),
         back_idx=back_idx.astype(np.int32), back_w=back_w)
-    return dict(host=host, M=M, Q=Q, G=G, N=N, n_levels=nlev, mgrid=mgrid)
+    return dict(host=host, M=M, Q=Q, G=G, N=N, n_levels=nlev, max_level_width=max_level_width, n_ovf=n_ovf,
+                mgrid=mgrid)
 
 
 class ScatterPlan:
@@ -175,6 +180,8 @@ class ScatterPlan:
         g = build_geometry(mtot, mrat, refine) if geom is None else geom
         self.M, self.Q, self.G, self.N = g['M'], g['Q'], g['G'], g['N']
         self.n_levels = g['n_levels']
+        self.max_level_width = g['max_level_width']
+        self.n_ovf = g['n_ovf']
         self.host = g['host']
         self.mgrid = g['mgrid']
         self._dev = None
@@ -281,25 +288,33 @@ def get_plan(mtot, mrat):
     return plan
 
 
-def add_scatter_to_masses_device(mtot, mrat, dens, scatter, tol=1e-6, maxiter=400):
+def add_scatter_to_masses_device(mtot, mrat, dens, scatter, tol=1e-6, maxiter=400, return_grads=False):
     """In place on the (M, Q, Z) device density; returns the per-slice Gauss-Seidel sweep counts (0 = no
-    convergence within maxiter, as the reference's estimator reports)."""
+    convergence within maxiter, as the reference's estimator reports).  ``return_grads``: also the
+    Clough-Tocher node gradients of every slice, (Z, N, 2) on the device (scipy's
+    ``CloughTocher2DInterpolator(...).grad`` of that slice, nodes in C order of (M, Q))."""
     plan = get_plan(mtot, mrat)
     d = dict(plan.device())
     d['P'] = plan.P_for(scatter)
     M, Q, Z = dens.shape
     assert (M, Q) == (plan.M, plan.Q)
-    ws = D.workspace(_lib.size("hd_scatter_workspace_size", M, Q, Z, plan.G))
+    ws = D.workspace(_lib.size("hd_scatter_workspace_size", M, Q, Z, plan.G, plan.n_ovf))
     iters = D.empty(Z + 1, dtype=torch.int32)
     from .engine import _stage
     with _stage("scatter"):
         _scatter_call(dens, M, Q, Z, plan, d, tol, maxiter, ws, iters)
-    return iters
+    if not return_grads:
+        return iters
+    # the gradients are the workspace's first region, at its first 256-byte boundary
+    off = (-ws.data_ptr()) % 256
+    grads = ws[off:off + Z * plan.N * 16].view(torch.float64).view(Z, plan.N, 2)
+    return iters, grads
 
 
 def _scatter_call(dens, M, Q, Z, plan, d, tol, maxiter, ws, iters):
     _lib.call("hd_scatter", D.ptr(dens), M, Q, Z, plan.G, D.ptr(d['pts']), D.ptr(d['nbr_ptr']), D.ptr(d['nbr_idx']),
-              D.ptr(d['levels_ptr']), D.ptr(d['levels_idx']), plan.n_levels, D.ptr(d['simplices']),
+              D.ptr(d['levels_ptr']), D.ptr(d['levels_idx']), plan.n_levels, plan.max_level_width, plan.n_ovf,
+              D.ptr(d['simplices']),
               D.ptr(d['simplex_g']), D.ptr(d['q_simplex']), D.ptr(d['q_bary']), D.ptr(d['q_nearest']),
               D.ptr(d['P']), D.ptr(d['back_idx']), D.ptr(d['back_w']), float(tol), int(maxiter), D.ptr(ws),
               ws.numel(), D.ptr(iters), D.stream_ptr())

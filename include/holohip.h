@@ -87,17 +87,23 @@ int hd_zero_stalled(double *dens, const double *zprime, int64_t n, void *stream)
  *   pts (N,2) node coordinates (log10 m1, log10 m2), N = M*Q, C order;
  *   nbr_ptr (N+1) / nbr_idx: vertex adjacency of the Delaunay triangulation;
  *   levels_ptr (n_levels+1) / levels_idx (N): Gauss-Seidel level schedule;
+ *   max_level_width: the most vertices of one level (at most 128);
+ *   n_ovf: adjacency entries beyond the eighth of each vertex, summed;
  *   simplices (S,3), simplex_g (S,3) cross-boundary derivative directions;
  *   q_simplex (G*G) (-1 outside the hull), q_bary (G*G,3), q_nearest (G*G);
  *   P (G,G) = (W W)^T for the Gaussian-CDF weight matrix W (see hd_gemm_f64);
  *   back_idx (N,4) regular-grid points, back_w (N,4) bilinear weights.
  * iters_out: Z+1 int32 -- Gauss-Seidel sweeps per slice (0: no convergence
  * within maxiter) and, last, the count of values still NaN/negative after the
- * nearest-node fill (the reference raises in that case, sam.py:1377-1383).
+ * nearest-node fill (the reference raises in that case, sam.py:1377-1383),
+ * with bit 20 set when the adjacency has more than n_ovf such entries and
+ * bit 21 when a level has more than max_level_width vertices (nothing
+ * computed then).
  * ---------------------------------------------------------------------- */
-size_t hd_scatter_workspace_size(int M, int Q, int Z, int G);
+size_t hd_scatter_workspace_size(int M, int Q, int Z, int G, int n_ovf);
 int hd_scatter(double *dens, int M, int Q, int Z, int G, const double *pts, const int32_t *nbr_ptr,
                const int32_t *nbr_idx, const int32_t *levels_ptr, const int32_t *levels_idx, int n_levels,
+               int max_level_width, int n_ovf,
                const int32_t *simplices, const double *simplex_g, const int32_t *q_simplex, const double *q_bary,
                const int32_t *q_nearest, const double *P, const int32_t *back_idx, const double *back_w, double tol,
                int maxiter, void *workspace, size_t ws_bytes, int32_t *iters_out, void *stream);

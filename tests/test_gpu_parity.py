@@ -162,6 +162,41 @@ def test_scatter_and_density_full_config1(gpu, oracle_lib):
     _close(dens, do, 1e-8, atol=1e-13 * scale, what="scattered density")
 
 
+def test_ct_gradients_bit_exact_vs_scipy(gpu):
+    """The Gauss-Seidel gradient estimate of every slice equals scipy's CloughTocher2DInterpolator
+    gradients (tol 1e-6, maxiter 400) bit for bit, at the full 91 x 81 mass grid: five slices of the
+    static density, one with a zeroed quadrant (zero numerators, signed zeros), and the density scaled
+    to 1e-295 and 1e280 (the divisions' exact fallback outside the fast step's range)."""
+    import scipy.interpolate
+    import torch
+    from holodeck_amd import device as D, scatter
+    spec = configs.spec_config2()
+    sam = configs.make_sam(spec, scatter=0.0)
+    dens0, _, _ = sam._density_device()
+    base = D.to_host(dens0)
+    Zall = base.shape[-1]
+    sl = [base[:, :, z] for z in (0, Zall // 4, Zall // 2, 3 * Zall // 4, Zall - 1)]
+    zq = base[:, :, Zall // 2].copy()
+    zq[: zq.shape[0] // 2, : zq.shape[1] // 2] = 0.0
+    sl += [zq, base[:, :, Zall // 2] * 1e-295, base[:, :, Zall // 2] * 1e280]
+    dens = np.stack(sl, axis=-1)
+    print(f"[ct gradients {dens.shape}]", flush=True)
+    d = D.to_dev(dens)
+    iters, grads = scatter.add_scatter_to_masses_device(sam.mtot, sam.mrat, d, 0.28, return_grads=True)
+    torch.cuda.synchronize()
+    g = D.to_host(grads)
+    it = D.to_host(iters)
+    m1 = sam.mtot[:, None] / (1.0 + sam.mrat[None, :])
+    m2 = sam.mtot[:, None] - m1
+    pts = (np.log10(m1.ravel()), np.log10(m2.ravel()))
+    for z in range(dens.shape[-1]):
+        want = np.asarray(scipy.interpolate.CloughTocher2DInterpolator(pts, dens[:, :, z].ravel()).grad)
+        want = want.reshape(-1, 2)
+        diff = np.flatnonzero(g[z].view(np.int64) != want.view(np.int64))
+        assert diff.size == 0, (z, it[z], diff.size, diff[:4], g[z].ravel()[diff[:4]], want.ravel()[diff[:4]])
+        assert it[z] > 0
+
+
 def test_stageA_full_config1_with_scatter(gpu, oracle_lib):
     """number and h2fdf at the full config-1 workload (61x51x41, 20 frequencies, GW-only)."""
     print("[stageA full config1]", flush=True)
