@@ -575,6 +575,30 @@ __device__ __noinline__ void gz_build(gz_lds_u32* hi, double lam, int kb, int kt
     }
 }
 
+// A short table built by one thread in one pass: the same entries as gz_build
+// (the CDF summed from kb by the pmf recurrence, floor(2^64 CDF) as high and low
+// words), the sum running in order, so the entries are non-decreasing as they
+// are.  For tables whose counts kb .. kt + n - 1 number at most GZ_SEQ_MAX: a
+// thread per table avoids the wave scans, which cost more than such a table.
+constexpr int GZ_SEQ_MAX = 32;
+__device__ __forceinline__ void gz_build_seq(uint32_t* hi, double lam, int kb, int kt, int n, double p0) {
+    uint32_t* lo = hi + n;
+    double p = p0, c = 0.0;
+    int k = kb;
+    for (; k < kt; k++) {   // the lower tail, summed into entry 0
+        c += p;
+        p *= lam * gz_rcp((double)(k + 1));
+    }
+    for (int j = 0; j < n; j++, k++) {
+        c += p;
+        uint32_t h, l;
+        gz_fixed64(c, h, l);
+        hi[j] = h;
+        lo[j] = l;
+        p *= lam * gz_rcp((double)(k + 1));
+    }
+}
+
 // Inversion by the stream: the smallest count k with CDF(k) > u, u = (x, w) / 2^64
 // for the next words x, w.  A branch-free binary search over the table's high
 // words finds the first entry j with hi[j] >= x; when that high word equals x the
@@ -688,6 +712,13 @@ __device__ __forceinline__ int gz_draw1(const GzTab& t, Words& rs) {
     const uint32_t x = rs.next();
     gz_search_start(s, t.hi, t.n, x);
     while (s.len > 1) gz_search_step(s);
+    {   // the common case, as in gz_draw2
+        const uint32_t h0 = s.p[0], h1 = s.p[1];
+        const bool up = h0 < x;
+        const int j = (int)(s.p - s.hi) + (up ? 1 : 0);
+        const bool slow = (j < t.n && (up ? h1 : h0) == x) || (j == 0 && t.kt > t.kb) || j == t.n;
+        if (__ballot(slow) == 0ull) return t.kt + j;
+    }
     uint32_t w = 0;
     bool hw;
     const int j = gz_search_finish(s, rs, w, hw);
@@ -719,6 +750,20 @@ __device__ __forceinline__ void gz_draw2(const GzTab& ta, const GzTab& tb, Words
     }
     while (a.len > 1) gz_search_step(a);
     while (b.len > 1) gz_search_step(b);
+    {   // the common case: no lane of the wave has a tie on a high word (the low words needed)
+        // or a draw outside a table, so every count is kt + j from the candidates alone (one
+        // wave-wide test instead of per-lane branches); otherwise the exact path below
+        const uint32_t a0 = a.p[0], a1 = a.p[1], b0 = b.p[0], b1 = b.p[1];
+        const bool ua = a0 < xa, ub = b0 < xb;
+        const int ja = (int)(a.p - a.hi) + (ua ? 1 : 0), jb = (int)(b.p - b.hi) + (ub ? 1 : 0);
+        const bool sa = (ja < ta.n && (ua ? a1 : a0) == xa) || (ja == 0 && ta.kt > ta.kb) || ja == ta.n;
+        const bool sb = (jb < tb.n && (ub ? b1 : b0) == xb) || (jb == 0 && tb.kt > tb.kb) || jb == tb.n;
+        if (__ballot((ta.n > 0 && sa) || (tb.n > 0 && sb)) == 0ull) {
+            ca = (ta.n > 0) ? ta.kt + ja : 0;
+            cb = (tb.n > 0) ? tb.kt + jb : 0;
+            return;
+        }
+    }
     uint32_t wa = 0, wb = 0;
     bool hwa = false, hwb = false;
     const int ja = (ta.n > 0) ? gz_search_finish(a, rs, wa, hwa) : 1;
@@ -774,7 +819,9 @@ template <int MODE, int LM, int NP, bool REC>
 // (6 waves per SIMD: with the LDS below three 512-thread workgroups share a CU)
 __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
     extern __shared__ double gz_lds[];
-    __shared__ int s_next;             // next record of the batch to build
+    __shared__ int s_next;             // next long table of the batch to build
+    __shared__ int s_nlarge;           // long tables of the batch
+    __shared__ int s_large[GZ_BATCH_REC];
     uint32_t* tab = (uint32_t*)gz_lds;   // A.cap 8-byte slots: per record n high words, then n low words
     double* scum = gz_lds + A.cap;     // GZ_TILE doubles
     BRec* brec = (BRec*)(scum + GZ_TILE);   // GZ_BATCH_REC staged record headers
@@ -819,26 +866,42 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
                 }
                 c1 = max(c1, c + 1);   // every window fits alone (n <= cap); keeps progress unconditional
                 c1 = min(c1, c + GZ_BATCH_REC);
-                if (tid == 0) s_next = c;
+                if (tid == 0) {
+                    s_next = 0;
+                    s_nlarge = 0;
+                }
                 __syncthreads();   // the previous batch's draws are done with tab / scum
-                // waves take the batch's records one at a time (windows differ in length); the
-                // next record's header is loaded while the current one's table is built
+                // one thread per record of the batch stages its header and builds a short table
+                // itself, in one sequential pass; longer tables are listed for the waves
+                {   // record i of the batch on wave i % 8, lane i / 8: every wave takes a share
+                    const int i = (tid & 63) * 8 + (tid >> 6);
+                    if (c + i < c1) {
+                        const DRec rec = A.drec[rreg + c + i];
+                        brec[i] = BRec{rec.h, rec.word, rec.lam, rec.kt, rec.n, rec.off - base, rec.kb};
+                        if (rec.n > 0 && rec.kt + rec.n - rec.kb <= GZ_SEQ_MAX)
+                            gz_build_seq(tab + 2 * (rec.off - base), rec.lam, rec.kb, rec.kt, rec.n, rec.p0);
+                        else if (rec.n > 0)
+                            s_large[atomicAdd(&s_nlarge, 1)] = i;
+                    }
+                }
+                __syncthreads();
+                // waves take the long tables one at a time (their lengths differ); the next
+                // record's header is loaded while the current one's table is built
                 {
+                    const int nl = s_nlarge;
                     int i = 0;
                     if (lane == 0) i = atomicAdd(&s_next, 1);
                     i = __shfl(i, 0, 64);
                     DRec rec;
-                    if (i < c1) rec = A.drec[rreg + i];
-                    while (i < c1) {
+                    if (i < nl) rec = A.drec[rreg + c + s_large[i]];
+                    while (i < nl) {
                         int inext = 0;
                         if (lane == 0) inext = atomicAdd(&s_next, 1);
                         inext = __shfl(inext, 0, 64);
                         DRec nrec;
-                        if (inext < c1) nrec = A.drec[rreg + inext];
-                        if (lane == 0) brec[i - c] = BRec{rec.h, rec.word, rec.lam, rec.kt, rec.n, rec.off - base, rec.kb};
-                        if (rec.n > 0)
-                            gz_build((gz_lds_u32*)(tab + 2 * (rec.off - base)), rec.lam, rec.kb, rec.kt, rec.n, rec.p0,
-                                     lane);
+                        if (inext < nl) nrec = A.drec[rreg + c + s_large[inext]];
+                        gz_build((gz_lds_u32*)(tab + 2 * (rec.off - base)), rec.lam, rec.kb, rec.kt, rec.n, rec.p0,
+                                 lane);
                         i = inext;
                         rec = nrec;
                     }
@@ -945,7 +1008,13 @@ __global__ void __launch_bounds__(64) k_cdf_table_hook(double lam, int cap, int*
     const int lane = threadIdx.x;
     int kb, kt, n;
     gz_window(lam, cap, kb, kt, n);
-    if (n > 0) gz_build((gz_lds_u32*)tab, lam, kb, kt, n, gz_pmf_acc(lam, (double)kb), lane);
+    if (n > 0) {   // the build the draw kernel uses for this length
+        if (kt + n - kb <= GZ_SEQ_MAX) {
+            if (lane == 0) gz_build_seq(tab, lam, kb, kt, n, gz_pmf_acc(lam, (double)kb));
+        } else {
+            gz_build((gz_lds_u32*)tab, lam, kb, kt, n, gz_pmf_acc(lam, (double)kb), lane);
+        }
+    }
     __syncthreads();
     for (int j = lane; j < n; j += 64) cdf_out[j] = ((uint64_t)tab[j] << 32) | tab[n + j];
     if (lane == 0) {
