@@ -225,12 +225,13 @@ __global__ void __launch_bounds__(256) k_dbn_gw(EvoConsts K, const double* __res
 // steps in LDS (separation chain, rest-frame frequency, dt, cumulative time;
 // the chain and the cumulative sum are sequential exactly as in the
 // reference).  Then
-//   phase 1: each thread owns one redshift corner kk and walks the steps,
-//            bracketing the target frequencies; for every target it records
-//            the LAST step that brackets it (the reference's last-write-wins
-//            per (kk, ff)), honouring the reference's break when the
-//            interpolated time passes the age of the universe.  The step index
-//            is parked in redz_final (the workgroup owns those rows);
+//   phase 1: a few threads per redshift corner kk each walk a contiguous share
+//            of the steps, bracketing the target frequencies; every target
+//            keeps the LARGEST step that brackets it (an atomic max: the
+//            reference's last write over its in-order steps, per (kk, ff)),
+//            honouring the reference's break when the interpolated time passes
+//            the age of the universe.  The steps are kept in LDS (or, when
+//            Z x F does not fit, in redz_final, which the workgroup owns);
 //   phase 2: one thread per (kk, ff) recomputes that step's bracket and does
 //            the per-target work (Kepler separation, hardening rate, residence
 //            time, cosmological factor) with every lane busy.
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(256) k_dbn_gw(EvoConsts K, const double* __res
 // phase 2 compute exactly that interval).
 // ---------------------------------------------------------------------------
 constexpr int DBN_MAX_STEPS = 4096;
-constexpr int DBN_THREADS = 128;
+constexpr int DBN_THREADS = 256;
 
 // first index i in [0, n-2] with edges[i+1] >= val (n-2 if none): the
 // interval a forward walk of while_while_increasing stops at
@@ -256,6 +257,7 @@ __device__ __forceinline__ int bracket_increasing(int n, double val, const doubl
 
 constexpr int DBN_FOBS_LDS = 1024;   // target frequencies staged in LDS up to this many
 
+template <bool RF_LDS>
 __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const double* __restrict__ fobs, int F,
                                                           int sorted_f, double sepa_init, int nsteps,
                                                           const double* __restrict__ hard_norm, double rchar,
@@ -279,6 +281,7 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
     double* s_gage = s_gdc + n_interp;
     double* s_agek = s_gage + n_interp;   // (Z)
     double* s_fobs = s_agek + Z;          // (F) when F <= DBN_FOBS_LDS
+    int* s_rf = (int*)(s_fobs + (F <= DBN_FOBS_LDS ? F : 0));   // (Z, F) recorded steps when RF_LDS
 
     const int ii = blockIdx.x / Q;
     const int jj = blockIdx.x % Q;
@@ -317,13 +320,20 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
         s_dadt[s] = hard_func_2pwl_gw(K, mt, mr, sp, norm, rchar, g_in, g_out);
         s_sepa[s] = sp;   // each s is read and written by the same thread
     }
-    // age at each redshift edge from the table (sam_cyutils.pyx:586-601)
+    // age at each redshift edge from the table (sam_cyutils.pyx:586-601): the reference walks the
+    // decreasing redshift nodes from the start while gz[ia + 1] > zk; the same index by a binary
+    // search for the first node at or below zk
     for (int kk = tid; kk < Z; kk += blockDim.x) {
         const double zk = redz[kk];
-        int ia = 0;
-        while ((ia + 1 < n_interp) && (gz_g[ia + 1] > zk)) ia++;
+        int lo = 1, hi = n_interp;   // first j in [1, n) with gz[j] <= zk, n if none
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_gz[mid] > zk) lo = mid + 1;
+            else hi = mid;
+        }
+        int ia = lo - 1;
         if (ia > n_interp - 2) ia = n_interp - 2;
-        s_agek[kk] = interp_between(zk, gz_g[ia], gz_g[ia + 1], gage_g[ia], gage_g[ia + 1]);
+        s_agek[kk] = interp_between(zk, s_gz[ia], s_gz[ia + 1], s_gage[ia], s_gage[ia + 1]);
     }
     __syncthreads();
     for (int s = tid + 1; s <= nsteps; s += blockDim.x)
@@ -340,59 +350,108 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
     __syncthreads();
 
     const double age_universe = s_gage[n_interp - 1];
-    // phase 1: the bracketing step of every (kk, ff), parked in redz_final
-    for (int kk = tid; kk < Z; kk += blockDim.x) {
-        double* rf = rf0 + (int64_t)kk * F;
-        for (int ff = 0; ff < F; ff++) rf[ff] = -1.0;
+    // the step recorded per (kk, ff): the largest that brackets the target, as the reference's
+    // last write over its in-order steps; -1 none.  In LDS when it fits, else in redz_final as
+    // 64-bit step + 1 (0 none)
+    const int nzf = Z * F;
+    unsigned long long* rfg = (unsigned long long*)rf0;
+    for (int p = tid; p < nzf; p += blockDim.x) {
+        if (RF_LDS) s_rf[p] = -1;
+        else rfg[p] = 0ull;
+    }
+    __syncthreads();
+    // phase 1: S threads per redshift corner kk, each walking a contiguous share of the steps
+    // (its first table interval by a binary search: the interval a walk from the start reaches)
+    const int S = max(1, (int)blockDim.x / Z);
+    for (int t = tid; t < Z * S; t += blockDim.x) {
+        const int kk = t / S, seg = t - kk * S;
+        const int st0 = (int)((int64_t)nsteps * seg / S), st1 = (int)((int64_t)nsteps * (seg + 1) / S);
         const double age_k = s_agek[kk];
         const double gmt = gmt_time[row0 + kk];
-        int idx_left = 0;
-        for (int step = 0; step < nsteps; step++) {
+        int idx_left = 0, ff_lo = 0;
+        if (seg > 0) idx_left = bracket_increasing(n_interp, s_tevo[st0 + 1] + gmt + age_k - s_dt[st0 + 1], s_gage);
+        // The nodes of the last walks' intervals are kept in registers: a walk that would not move
+        // (the time inside the cached interval) reads nothing.  Left: the reference's walk from the
+        // previous interval stays put exactly when gage[il] <= t <= gage[il + 1].  Right: the walk
+        // from idx_left ends in the left interval when t <= gage[idx_left + 1], in the cached right
+        // interval ir >= idx_left when gage[ir] < t <= gage[ir + 1].  Targets: the first target
+        // >= fl stays put when fo[ff_lo - 1] < fl <= fo[ff_lo].
+        int il = -1, ir = -1;
+        double gl0 = 0.0, gl1 = 0.0, zl0 = 0.0, zl1 = 0.0, gr0 = 0.0, gr1 = 0.0, zr0 = 0.0, zr1 = 0.0;
+        double f_below = -1.0, f_at = -1.0;   // fo[ff_lo - 1], fo[ff_lo] (+-inf past the ends); unset
+        bool f_set = false;
+        for (int step = st0; step < st1; step++) {
             const double dt = s_dt[step + 1];
             const double time_right = s_tevo[step + 1] + gmt + age_k;
             const double time_left = time_right - dt;
             if (time_left > age_universe) continue;
-            idx_left = ww_increasing(idx_left, n_interp, time_left, s_gage);
-            const double redz_left = interp_at_index(idx_left, time_left, s_gage, s_gz);
+            if (!(il >= 0 && gl0 <= time_left && time_left <= gl1)) {
+                idx_left = ww_increasing(idx_left, n_interp, time_left, s_gage);
+                il = idx_left;
+                gl0 = s_gage[il];
+                gl1 = s_gage[il + 1];
+                zl0 = s_gz[il];
+                zl1 = s_gz[il + 1];
+            }
+            const double redz_left = interp_between(time_left, gl0, gl1, zl0, zl1);
             if (redz_left < 0.0) continue;
-            const int idx_right = ww_increasing(idx_left, n_interp, time_right, s_gage);
-            double redz_right = interp_at_index(idx_right, time_right, s_gage, s_gz);
+            double redz_right;
+            if (time_right <= gl1) {
+                redz_right = interp_between(time_right, gl0, gl1, zl0, zl1);
+            } else {
+                if (!(ir >= idx_left && gr0 < time_right && time_right <= gr1)) {
+                    ir = ww_increasing(idx_left, n_interp, time_right, s_gage);
+                    gr0 = s_gage[ir];
+                    gr1 = s_gage[ir + 1];
+                    zr0 = s_gz[ir];
+                    zr1 = s_gz[ir + 1];
+                }
+                redz_right = interp_between(time_right, gr0, gr1, zr0, zr1);
+            }
             if (redz_right < 0.0) redz_right = 0.0;
             const double fl = s_frst[step] / (1.0 + redz_left);
             const double fr = s_frst[step + 1] / (1.0 + redz_right);
-            int ff = 0;
             if (sorted_f) {
-                // first target >= fl (targets are non-decreasing)
-                int lo = 0, hi = F;
-                while (lo < hi) {
-                    int mid = (lo + hi) >> 1;
-                    if (fo[mid] < fl) lo = mid + 1;
-                    else hi = mid;
+                // first target >= fl (targets are non-decreasing), moved from the previous step's:
+                // fl grows along the evolution, so it moves by a few places at most, and the two
+                // walks give the first such target for any fl
+                if (!(f_set && f_below < fl && fl <= f_at)) {
+                    while (ff_lo > 0 && fo[ff_lo - 1] >= fl) ff_lo--;
+                    while (ff_lo < F && fo[ff_lo] < fl) ff_lo++;
+                    f_below = (ff_lo > 0) ? fo[ff_lo - 1] : -HUGE_VAL;
+                    f_at = (ff_lo < F) ? fo[ff_lo] : HUGE_VAL;
+                    f_set = true;
                 }
-                ff = lo;
-            }
-            for (; ff < F; ff++) {
-                const double ft = fo[ff];
-                if ((ft < fl) || (fr < ft)) {
-                    if (sorted_f) break;   // past the contiguous bracketed range
-                    continue;
+                if (!(f_at <= fr)) continue;   // no target in [fl, fr]
+                for (int ff = ff_lo; ff < F; ff++) {
+                    const double ft = fo[ff];
+                    if (fr < ft) break;   // past the contiguous bracketed range
+                    const double new_time = interp_between(ft, fl, fr, time_left, time_right);
+                    if (new_time > age_universe) break;
+                    if (RF_LDS) atomicMax(s_rf + kk * F + ff, step);
+                    else atomicMax(rfg + (int64_t)kk * F + ff, (unsigned long long)step + 1ull);
                 }
-                const double new_time = interp_between(ft, fl, fr, time_left, time_right);
-                if (new_time > age_universe) break;
-                rf[ff] = (double)step;
+            } else {
+                for (int ff = 0; ff < F; ff++) {
+                    const double ft = fo[ff];
+                    if ((ft < fl) || (fr < ft)) continue;
+                    const double new_time = interp_between(ft, fl, fr, time_left, time_right);
+                    if (new_time > age_universe) break;
+                    if (RF_LDS) atomicMax(s_rf + kk * F + ff, step);
+                    else atomicMax(rfg + (int64_t)kk * F + ff, (unsigned long long)step + 1ull);
+                }
             }
         }
     }
-    __syncthreads();   // phase-1 rows (global memory, this workgroup's own) are complete
+    __syncthreads();   // every recorded step is in place
     // phase 2: the per-target work for every recorded (kk, ff)
-    const int nzf = Z * F;
     for (int p = tid; p < nzf; p += blockDim.x) {
-        const double sv = rf0[p];
-        if (sv < 0.0) {
+        const int step = RF_LDS ? s_rf[p] : (int)((long long)rfg[p] - 1);
+        if (step < 0) {
+            rf0[p] = -1.0;
             dn0[p] = 0.0;
             continue;
         }
-        const int step = (int)sv;
         const int kk = p / F, ff = p - (p / F) * F;
         const double gmt = gmt_time[row0 + kk];
         const double nd = nden[row0 + kk];
@@ -507,9 +566,17 @@ extern "C" int hd_dbn_2pwl(const double* fobs_orb, int F, int fobs_sorted, doubl
                         (F <= DBN_FOBS_LDS ? (size_t)F : 0)) * sizeof(double);
     if (lds > 65536)
         return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: step/cosmology tables need %zu B of LDS (> 64 KiB)", lds);
-    hipLaunchKernelGGL(k_dbn_2pwl, dim3(M * Q), dim3(DBN_THREADS), lds, (hipStream_t)stream, evo_consts(), fobs_orb, F,
-                       fobs_sorted, sepa_init, num_steps, hard_norm, rchar, gamma_inner, gamma_outer, nden, mtot, M,
-                       mrat, Q, redz, Z, gmt_time, grid_z, grid_dcom, grid_age, n_interp, redz_final, diff_num);
+    const size_t lds_rf = lds + (size_t)Z * F * sizeof(int);   // with the recorded steps
+    if (lds_rf <= 65536)
+        hipLaunchKernelGGL(k_dbn_2pwl<true>, dim3(M * Q), dim3(DBN_THREADS), lds_rf, (hipStream_t)stream, evo_consts(),
+                           fobs_orb, F, fobs_sorted, sepa_init, num_steps, hard_norm, rchar, gamma_inner, gamma_outer,
+                           nden, mtot, M, mrat, Q, redz, Z, gmt_time, grid_z, grid_dcom, grid_age, n_interp,
+                           redz_final, diff_num);
+    else
+        hipLaunchKernelGGL(k_dbn_2pwl<false>, dim3(M * Q), dim3(DBN_THREADS), lds, (hipStream_t)stream, evo_consts(),
+                           fobs_orb, F, fobs_sorted, sepa_init, num_steps, hard_norm, rchar, gamma_inner, gamma_outer,
+                           nden, mtot, M, mrat, Q, redz, Z, gmt_time, grid_z, grid_dcom, grid_age, n_interp,
+                           redz_final, diff_num);
     return check_launch("k_dbn_2pwl");
 }
 

@@ -40,17 +40,20 @@ def _stageA(spec, gpu, oracle_lib):
     return sam, hard, o, D.to_host(number), D.to_host(h2fdf), D.to_host(rf)
 
 
-@pytest.mark.parametrize("which", ["config1", "config2", "config2_many_f"])
+@pytest.mark.parametrize("which", ["config1", "config2", "config2_many_f", "config2_more_f"])
 def test_stageA_parity(which, gpu, oracle_lib):
     """config2_many_f: 101 redshift edges x 100 frequencies (long bracket walks, many targets
-    per step)."""
+    per step); config2_more_f: 200 frequencies, whose recorded steps (Z x F) no longer fit the
+    evolution kernel's LDS, so they are kept in global memory."""
     print(f"[stageA {which}]", flush=True)
     if which == "config1":
         spec = configs.spec_config1(shape=SMALL, nfreq=7)
     elif which == "config2":
         spec = configs.spec_config2(shape=SMALL, nfreq=7)
-    else:
+    elif which == "config2_many_f":
         spec = configs.spec_config2(shape=(7, 6, 101), nfreq=100)
+    else:
+        spec = configs.spec_config2(shape=(7, 6, 101), nfreq=200)
     sam, hard, o, number, h2fdf, rf = _stageA(spec, gpu, oracle_lib)
     _close(sam.static_binary_density, o['dens'], 1e-12, what="density")
     if 'norm_log10' in o:
