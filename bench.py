@@ -17,7 +17,8 @@ with and without the judge, the judged median error in dex, the plain SAM's
 spectra/s at the surrogate's configuration (from the judge runs), and the MFMA
 roofline fraction of the inference GEMMs.  An eccentric-SAM line (61x51x61 grid,
 40 frequencies, 100 harmonics; deterministic and 200-realization discrete
-spectra/s) follows.
+spectra/s) and a batched-library line (256 of 2048 Latin-hypercube samples at
+their own uneven grids, 30 frequencies, 100 realizations: samples/s) follow.
 
 Prints one JSON line (rank 0).  --gpus N>1 is rejected: this build runs on one GPU.
 """
@@ -36,6 +37,10 @@ sys.path.insert(0, ROOT)
 METRIC = "GWB spectra/sec on 1 MI355X: plain GPU SAM vs MLP surrogate at ≤0.05 dex err"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_hd_realize_91x81x101_f40_r500.json")
+# wave-level VALU instructions of one k_gz_draw launch (rocprofv3 SQ_INSTS_VALU), for the issue roofline
+VALU_FILE = os.path.join(ROOT, "profiles", "pmc_valu_k_gz_draw_91x81x101_f40_r500_l10.json")
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per SIMD per 4 cycles
+VALU_PEAK_GINST_S = 256 * 4 * 2.4 / 4
 SURROGATE_WEIGHTS = os.path.join(ROOT, "holodeck_amd", "data", "surrogate_classic_uniform_91x81x101_f30_r500.npz")
 BF16_DENSE_PEAK_TFS = 2500.0
 CALIB_BYTES = 1 << 30   # copy-kernel calibration size for the PMC passes (bytes read = bytes written)
@@ -192,6 +197,35 @@ def surrogate_bench(n, judge_frac, seed=7):
     }
 
 
+def library_bench(nsamples=256, nreals=100, nfreqs=30, streams=3, nwarm=16):
+    """Batched library: nsamples of 2048 Latin-hypercube samples of the classic uniform space, each
+    at its own grid drawn from a seed (41-91 x 41-81 x 41-101 edges), 30 frequencies, 100
+    realizations, samples pipelined over HIP streams (scatter geometry built ahead by host worker
+    processes).  The last nwarm samples run first, untimed (they start the worker pool); the timed
+    call covers the first nsamples, whose grids and geometry are all new to it.  Wall clock of the
+    timed call: samples/s."""
+    from holodeck_amd import librarian as L
+    S_all = 2048
+    ps = L.PS_Classic_Phenom_Uniform(nsamples=S_all, seed=12345)
+    shapes_all = L.sample_shapes(S_all, 777)
+    warm = np.arange(S_all - nwarm, S_all)
+    L.gwb_library(ps, shapes=shapes_all[warm], nfreqs=nfreqs, nreals=nreals, seed=3, samples=warm, streams=streams)
+    rows = np.arange(nsamples)
+    shapes = shapes_all[rows]
+    t0 = time.perf_counter()
+    lib = L.gwb_library(ps, shapes=shapes, nfreqs=nfreqs, nreals=nreals, seed=3, samples=rows, streams=streams)
+    dt = time.perf_counter() - t0
+    assert np.all(np.isfinite(lib['gwb'])) and not lib['failed'].any()
+    cells = float(np.sum(np.prod(shapes - 1, axis=1)))
+    return {"config": {"workload": (f"batched library: {nsamples} of 2048 Latin-hypercube samples of the 6-D classic "
+                                    "uniform space, each at its own grid (41-91 x 41-81 x 41-101 edges), "
+                                    f"{nfreqs} frequencies, {nreals} realizations, {streams} HIP streams; "
+                                    f"{nwarm} other samples first, untimed"),
+                       "nsamples": nsamples, "nfreqs": nfreqs, "nreals": nreals, "streams": streams},
+            "samples_per_s": nsamples / dt, "wall_s": dt,
+            "cell_realizations_per_s": cells * nfreqs * nreals / dt}
+
+
 def eccentric_bench(e0=0.5, nreals=200, nharms=100):
     """Eccentric SAM on the 61x51x61 grid, 40 frequencies i/T, 100 harmonics: the deterministic
     harmonic sum and the discrete variant with nreals Poisson realizations, one spectrum each."""
@@ -242,6 +276,8 @@ def main():
                     help="single-stream steps after the timed region that time the realization launch")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-eccentric", action="store_true")
+    ap.add_argument("--no-library", action="store_true")
+    ap.add_argument("--library-n", type=int, default=256)
     ap.add_argument("--surrogate-n", type=int, default=100000)
     ap.add_argument("--judge-frac", type=float, default=0.01)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -325,6 +361,18 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             traffic = json.load(fh).get("traffic_bytes_per_launch")
+    # the draw kernel is bound by instruction issue, not HBM: its VALU instructions per launch
+    # (counted by rocprofv3 on the same workload) over its live-timed duration, against the peak
+    valu = None
+    t_draw = draw_ms / max(draw_n, 1) * 1e-3
+    if os.path.exists(VALU_FILE) and t_draw > 0:
+        with open(VALU_FILE) as fh:
+            vi = json.load(fh).get("valu_insts_per_launch")
+        if vi:
+            ach = vi / t_draw / 1e9
+            valu = {"bound": "valu_issue", "kernel": "k_gz_draw", "achieved": ach, "peak": VALU_PEAK_GINST_S,
+                    "unit": "G wave-instructions/s", "frac": ach / VALU_PEAK_GINST_S, "insts_per_launch": vi,
+                    "launch_ms": t_draw * 1e3}
     line = {
         "metric": METRIC, "value": value, "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -341,12 +389,15 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "alg_bytes_per_launch": alg_bytes, "launch_ms": t_real * 1e3,
                      "launches": real_n, "k_gz_draw_ms": draw_ms / max(draw_n, 1),
-                     "timed_on": "timed steps" if len(streams) == 1 else f"{args.roofline_steps} single-stream steps"},
+                     "timed_on": "timed steps" if len(streams) == 1 else f"{args.roofline_steps} single-stream steps",
+                     "valu": valu},
     }
     if not args.no_surrogate and os.path.exists(SURROGATE_WEIGHTS):
         line["surrogate"] = surrogate_bench(args.surrogate_n, args.judge_frac)
     if not args.no_eccentric:
         line["eccentric"] = eccentric_bench()
+    if not args.no_library:
+        line["library"] = library_bench(args.library_n)
     if cpu is not None:
         line["cpu_baseline"] = cpu
         line["x_vs_cpu_baseline"] = value / cpu["value"]

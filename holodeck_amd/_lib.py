@@ -43,6 +43,8 @@ SIGNATURES = {
     "hd_scatter": (_I32, [_P, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P,
                           _P, _P, _D, _I32, _P, _SZ, _P, _P]),
     "hd_gemm_f64": (_I32, [_P, _P, _P, _I32, _I32, _I32, _P]),
+    "hd_ct_locate": (_I32, [_P, _I32, _P, _P, _I32, _P, _I32, _P, _P, _P, _P]),
+    "hd_ct_levels": (_I32, [_P, _P, _I32, _P]),
     "hd_scatter_weights_T": (_I32, [_P, _I32, _P, _P]),
     "hd_bin_params_redz":(_I32, [_P, _I32, _P, _I32, _P, _I32, _P, _I32, _P, _D, _D, _P, _P, _I32, _P, _P]),
     "hd_order_workspace_size": (_SZ, [_I64]),

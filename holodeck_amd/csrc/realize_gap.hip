@@ -93,6 +93,7 @@ struct BRec {
     uint64_t word;
     double lam;
     int32_t kt, n, off, kb;
+    double p0;   // pmf at kb (the build's start)
 };
 
 struct GapArgs {
@@ -877,7 +878,7 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
                     const int i = (tid & 63) * 8 + (tid >> 6);
                     if (c + i < c1) {
                         const DRec rec = A.drec[rreg + c + i];
-                        brec[i] = BRec{rec.h, rec.word, rec.lam, rec.kt, rec.n, rec.off - base, rec.kb};
+                        brec[i] = BRec{rec.h, rec.word, rec.lam, rec.kt, rec.n, rec.off - base, rec.kb, rec.p0};
                         if (rec.n > 0 && rec.kt + rec.n - rec.kb <= GZ_SEQ_MAX)
                             gz_build_seq(tab + 2 * (rec.off - base), rec.lam, rec.kb, rec.kt, rec.n, rec.p0);
                         else if (rec.n > 0)
@@ -885,25 +886,17 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
                     }
                 }
                 __syncthreads();
-                // waves take the long tables one at a time (their lengths differ); the next
-                // record's header is loaded while the current one's table is built
+                // waves take the long tables one at a time (their lengths differ), from the
+                // staged headers
                 {
                     const int nl = s_nlarge;
-                    int i = 0;
-                    if (lane == 0) i = atomicAdd(&s_next, 1);
-                    i = __shfl(i, 0, 64);
-                    DRec rec;
-                    if (i < nl) rec = A.drec[rreg + c + s_large[i]];
-                    while (i < nl) {
-                        int inext = 0;
-                        if (lane == 0) inext = atomicAdd(&s_next, 1);
-                        inext = __shfl(inext, 0, 64);
-                        DRec nrec;
-                        if (inext < nl) nrec = A.drec[rreg + c + s_large[inext]];
-                        gz_build((gz_lds_u32*)(tab + 2 * (rec.off - base)), rec.lam, rec.kb, rec.kt, rec.n, rec.p0,
-                                 lane);
-                        i = inext;
-                        rec = nrec;
+                    for (;;) {
+                        int i = 0;
+                        if (lane == 0) i = atomicAdd(&s_next, 1);
+                        i = __shfl(i, 0, 64);
+                        if (i >= nl) break;
+                        const BRec* rec = brec + s_large[i];
+                        gz_build((gz_lds_u32*)(tab + 2 * rec->off), rec->lam, rec->kb, rec->kt, rec->n, rec->p0, lane);
                     }
                 }
                 if (c1 == nrec && ns > 0)
@@ -1038,7 +1031,7 @@ __global__ void __launch_bounds__(256) k_table_draw_hook(const uint32_t* tab, in
     uint32_t* t = (uint32_t*)hook_lds;
     __shared__ BRec hdr;
     for (int j = threadIdx.x; j < 2 * n; j += blockDim.x) t[j] = tab[j];
-    if (threadIdx.x == 0) hdr = BRec{0.0, 0, lam, kt, n, 0, kb};
+    if (threadIdx.x == 0) hdr = BRec{0.0, 0, lam, kt, n, 0, kb, 0.0};
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
@@ -1058,8 +1051,8 @@ __global__ void __launch_bounds__(256) k_table_draw_pair_hook(const uint32_t* ta
     __shared__ BRec hdr[2];
     for (int j = threadIdx.x; j < 2 * n2; j += blockDim.x) t2[j] = tab2[j];
     if (threadIdx.x == 0) {
-        hdr[0] = BRec{0.0, 0, lam, kt, n, 0, kb};
-        hdr[1] = BRec{0.0, 0, lam2, kt2, n2, 0, kb2};
+        hdr[0] = BRec{0.0, 0, lam, kt, n, 0, kb, 0.0};
+        hdr[1] = BRec{0.0, 0, lam2, kt2, n2, 0, kb2, 0.0};
     }
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

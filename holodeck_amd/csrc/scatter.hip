@@ -513,6 +513,108 @@ __global__ void k_zero_i32(int32_t* p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Location of the regular (G x G) query grid in the triangulation, on the device
+// (what the reference's CloughTocher2DInterpolator and NearestNDInterpolator look
+// up on the host: the containing simplex with its barycentric coordinates, and
+// the nearest node).  xi[i G + j] = (mgrid[i], mgrid[j]).
+// ---------------------------------------------------------------------------
+constexpr double CT_BARY_EPS = 100.0 * 2.220446049250313e-16;   // Qhull's inside test: c >= -eps, c <= 1 + eps
+
+// Barycentric coordinates of (x, y) in simplex s from its affine transform
+// T (S, 3, 2): c01 = T[:2] (xy - T[2]), c2 = 1 - (c0 + c1).
+__device__ __forceinline__ void ct_bary(const double* __restrict__ T, int s, double x, double y, double& c0,
+                                        double& c1, double& c2) {
+    const double* t = T + 6 * (int64_t)s;
+    const double dx = x - t[4], dy = y - t[5];
+    c0 = t[0] * dx + t[1] * dy;
+    c1 = t[2] * dx + t[3] * dy;
+    c2 = 1.0 - (c0 + c1);
+}
+
+// One thread per simplex: every grid point of its bounding box (+-2 nodes for the
+// grid's rounding) that passes the inside test takes the lowest such simplex
+// (a point on a shared edge may pass for both; the interpolant is continuous there).
+__global__ void __launch_bounds__(256) k_ct_raster(const double* __restrict__ pts, const int32_t* __restrict__ simp,
+                                                   const double* __restrict__ T, int S,
+                                                   const double* __restrict__ mgrid, int G,
+                                                   int32_t* __restrict__ q_simplex) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+    for (int k = 0; k < 3; k++) {
+        const int v = simp[3 * s + k];
+        xmin = fmin(xmin, pts[2 * v]);
+        xmax = fmax(xmax, pts[2 * v]);
+        ymin = fmin(ymin, pts[2 * v + 1]);
+        ymax = fmax(ymax, pts[2 * v + 1]);
+    }
+    const double g0 = mgrid[0], inv = (double)(G - 1) / (mgrid[G - 1] - g0);
+    const int i0 = max((int)floor((xmin - g0) * inv) - 2, 0), i1 = min((int)ceil((xmax - g0) * inv) + 2, G - 1);
+    const int j0 = max((int)floor((ymin - g0) * inv) - 2, 0), j1 = min((int)ceil((ymax - g0) * inv) + 2, G - 1);
+    for (int i = i0; i <= i1; i++)
+        for (int j = j0; j <= j1; j++) {
+            double c0, c1, c2;
+            ct_bary(T, s, mgrid[i], mgrid[j], c0, c1, c2);
+            const double lo = -CT_BARY_EPS, hi = 1.0 + CT_BARY_EPS;
+            if (c0 >= lo && c1 >= lo && c2 >= lo && c0 <= hi && c1 <= hi && c2 <= hi)
+                atomicMin(q_simplex + (int64_t)i * G + j, s);
+        }
+}
+
+__global__ void k_ct_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Per grid point: -1 where no simplex holds it, else its barycentric coordinates in
+// the chosen simplex; and the nearest node (squared distance, the lowest index on a
+// tie), the nodes staged through LDS in blocks.
+__global__ void __launch_bounds__(256) k_ct_bary_nearest(const double* __restrict__ pts, int N,
+                                                         const double* __restrict__ T,
+                                                         const double* __restrict__ mgrid, int G,
+                                                         int32_t* __restrict__ q_simplex, double* __restrict__ q_bary,
+                                                         int32_t* __restrict__ q_nearest) {
+    __shared__ double s_p[2 * 256];
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = g < (int64_t)G * G;
+    const double x = in ? mgrid[g / G] : 0.0, y = in ? mgrid[g % G] : 0.0;
+    if (in) {
+        const int s = q_simplex[g];
+        if (s == 0x7fffffff) {
+            q_simplex[g] = -1;
+            q_bary[3 * g] = q_bary[3 * g + 1] = q_bary[3 * g + 2] = 0.0;
+        } else {
+            double c0, c1, c2;
+            ct_bary(T, s, x, y, c0, c1, c2);
+            q_bary[3 * g] = c0;
+            q_bary[3 * g + 1] = c1;
+            q_bary[3 * g + 2] = c2;
+        }
+    }
+    double best = 1e308;
+    int bi = 0;
+    for (int base = 0; base < N; base += 256) {
+        __syncthreads();
+        const int v = base + threadIdx.x;
+        if (v < N) {
+            s_p[2 * threadIdx.x] = pts[2 * v];
+            s_p[2 * threadIdx.x + 1] = pts[2 * v + 1];
+        }
+        __syncthreads();
+        const int nb = min(256, N - base);
+        for (int k = 0; k < nb; k++) {
+            const double dx = x - s_p[2 * k], dy = y - s_p[2 * k + 1];
+            const double d = dx * dx + dy * dy;
+            if (d < best) {
+                best = d;
+                bi = base + k;
+            }
+        }
+    }
+    if (in) q_nearest[g] = bi;
+}
+
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace hd
@@ -636,4 +738,33 @@ extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double
     hipLaunchKernelGGL(k_back_interp, dim3((unsigned)cdiv((int64_t)N * Z, 256)), dim3(256), 0, st, Y, N, Z, G, back_idx,
                        back_w, dens);
     return check_launch("k_back_interp");
+}
+
+extern "C" int hd_ct_locate(const double* pts, int N, const int32_t* simplices, const double* transform, int S,
+                            const double* mgrid, int G, int32_t* q_simplex, double* q_bary, int32_t* q_nearest,
+                            void* stream) {
+    if (N < 3 || S < 1 || G < 2 || !pts || !simplices || !transform || !mgrid || !q_simplex || !q_bary || !q_nearest)
+        return set_error(HD_ERR_ARG, "hd_ct_locate: bad arguments");
+    if ((int64_t)G * G > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_ct_locate: G too large");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = (int64_t)G * G;
+    hipLaunchKernelGGL(k_ct_fill_i32, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, q_simplex, n, 0x7fffffff);
+    hipLaunchKernelGGL(k_ct_raster, dim3((unsigned)cdiv(S, 256)), dim3(256), 0, st, pts, simplices, transform, S, mgrid,
+                       G, q_simplex);
+    hipLaunchKernelGGL(k_ct_bary_nearest, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, pts, N, transform, mgrid, G,
+                       q_simplex, q_bary, q_nearest);
+    return check_launch("hd_ct_locate");
+}
+
+extern "C" int hd_ct_levels(const int32_t* nbr_ptr, const int32_t* nbr_idx, int N, int32_t* level) {
+    if (N < 1 || !nbr_ptr || !nbr_idx || !level) return set_error(HD_ERR_ARG, "hd_ct_levels: bad arguments");
+    for (int p = 0; p < N; p++) {   // one more than the highest level of the lower-indexed neighbours
+        int32_t l = 0;
+        for (int32_t k = nbr_ptr[p]; k < nbr_ptr[p + 1]; k++) {
+            const int32_t q = nbr_idx[k];
+            if (q >= 0 && q < p && level[q] + 1 > l) l = level[q] + 1;
+        }
+        level[p] = l;
+    }
+    return HD_OK;
 }

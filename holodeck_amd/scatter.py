@@ -65,7 +65,32 @@ def _rolled_weights(log_cents, scatter):
     return dm[k - j + num - 1]
 
 
-def _simplex_g(tri):
+def _transforms(pts, simplices):
+    """The affine barycentric transforms of scipy.spatial.Delaunay.transform (S, 3, 2): rows 0-1
+    the inverse of [v0 - v2, v1 - v2], row 2 the vertex v2 -- in closed form for 2x2 (the same
+    values to ~1e-14 relative; scipy solves each by LAPACK); NaN for a degenerate simplex, as
+    scipy marks one whose reciprocal condition number is below machine epsilon."""
+    P = pts[simplices]
+    a = P[:, 0, 0] - P[:, 2, 0]
+    b = P[:, 1, 0] - P[:, 2, 0]
+    c = P[:, 0, 1] - P[:, 2, 1]
+    d = P[:, 1, 1] - P[:, 2, 1]
+    det = a * d - b * c
+    T = np.empty((simplices.shape[0], 3, 2))
+    with np.errstate(divide='ignore', invalid='ignore'):
+        T[:, 0, 0] = d / det
+        T[:, 0, 1] = -b / det
+        T[:, 1, 0] = -c / det
+        T[:, 1, 1] = a / det
+    T[:, 2, :] = P[:, 2, :]
+    n1 = np.maximum(np.abs(a) + np.abs(c), np.abs(b) + np.abs(d))   # 1-norm of the matrix
+    ni = np.maximum(np.abs(d) + np.abs(c), np.abs(b) + np.abs(a)) / np.abs(det)
+    bad = ~(1.0 / (n1 * ni) >= np.finfo(float).eps)
+    T[bad] = np.nan
+    return T
+
+
+def _simplex_g(tri, T=None):
     """Affine-invariant cross-boundary directions of each triangle (Clough-Tocher; see
     P. Alfeld, CAGD 1 (1984) 169; G. Farin, CAGD 3 (1986) 83): for edge k (opposite
     vertex k) the direction is V4' - V4, the neighbour's centroid minus this
@@ -73,7 +98,7 @@ def _simplex_g(tri):
     S = tri.simplices
     P = tri.points
     nb = tri.neighbors
-    T = tri.transform
+    T = tri.transform if T is None else T
     g = np.full((S.shape[0], 3), -0.5)
     cent = P[S].mean(axis=1)
     for k in range(3):
@@ -94,27 +119,16 @@ def _simplex_g(tri):
     return g
 
 
-def _find_simplex(tri, pts, xi, margin=1e-6):
-    """``tri.find_simplex(xi)`` with the points clearly outside the convex hull (farther than
-    `margin` beyond a hull facet) set to -1 without a search.  Most regular-grid points lie
-    outside the hull of the (M, q) nodes, and Qhull's search is slowest for exactly those; the
-    points it still searches get the same answer as before."""
-    from scipy.spatial import ConvexHull
-    eq = ConvexHull(pts).equations                    # outward unit normals and offsets
-    dist = np.full(xi.shape[0], -np.inf)
-    for c in range(0, eq.shape[0], 64):
-        e = eq[c:c + 64]
-        dist = np.maximum(dist, (xi @ e[:, :2].T + e[:, 2]).max(axis=1))
-    near = dist <= margin
-    out = np.full(xi.shape[0], -1, dtype=np.int64)
-    out[near] = tri.find_simplex(xi[near])
-    return out
-
-
 def build_geometry(mtot, mrat, refine=4, kd_workers=-1):
-    """Host geometry of the scatter step for one (mtot, mrat) grid (numpy/scipy only, so it can be
-    built in a worker process).  Returns a dictionary of arrays and sizes for :class:`ScatterPlan`."""
-    from scipy.spatial import Delaunay, cKDTree
+    """Host geometry of the scatter step for one (mtot, mrat) grid (numpy/scipy and the library's
+    host routines only, so it can be built in a worker process): the Delaunay triangulation of the
+    (log10 m1, log10 m2) nodes, its level schedule, cross-boundary directions and affine
+    transforms, and the bilinear back-interpolation.  The regular grid's point location and
+    nearest nodes are made on the device from these (:meth:`ScatterPlan.device`).  Returns a
+    dictionary of arrays and sizes for :class:`ScatterPlan` (``kd_workers`` is accepted for
+    callers of the earlier host location and ignored)."""
+    import ctypes as C
+    from scipy.spatial import Delaunay
     mtot = np.asarray(mtot, dtype=float)
     mrat = np.asarray(mrat, dtype=float)
     M, Q = mtot.size, mrat.size
@@ -128,13 +142,13 @@ def build_geometry(mtot, mrat, refine=4, kd_workers=-1):
     mgrid = np.log10(np.logspace(*np.log10(mextr), G))
     tri = Delaunay(pts)
     indptr, indices = tri.vertex_neighbor_vertices
+    indptr = np.ascontiguousarray(indptr, dtype=np.int32)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
     N = pts.shape[0]
     # level schedule: a vertex may update once all lower-indexed neighbours have
-    level = np.zeros(N, dtype=np.int64)
-    for p in range(N):
-        nb = indices[indptr[p]:indptr[p + 1]]
-        lo_nb = nb[nb < p]
-        level[p] = (level[lo_nb].max() + 1) if lo_nb.size else 0
+    level = np.zeros(N, dtype=np.int32)
+    _lib.call("hd_ct_levels", indptr.ctypes.data_as(C.c_void_p), indices.ctypes.data_as(C.c_void_p), N,
+              level.ctypes.data_as(C.c_void_p))
     nlev = int(level.max()) + 1
     # the sweep runs eight lanes per vertex of a level; entries past a vertex's eighth go to an
     # overflow area
@@ -142,18 +156,6 @@ def build_geometry(mtot, mrat, refine=4, kd_workers=-1):
     n_ovf = int(np.maximum(np.diff(indptr) - 8, 0).sum())
     lorder = np.argsort(level, kind='stable')
     lptr = np.concatenate([[0], np.cumsum(np.bincount(level, minlength=nlev))])
-    # regular-grid query points xi[i*G + j] = (mgrid[i], mgrid[j])
-    X, Y = np.meshgrid(mgrid, mgrid, indexing='ij')
-    xi = np.stack([X.ravel(), Y.ravel()], axis=-1)
-    simp = _find_simplex(tri, pts, xi)
-    bary = np.zeros((xi.shape[0], 3))
-    ok = simp >= 0
-    Tm = tri.transform[simp[ok], :2, :]
-    r = tri.transform[simp[ok], 2, :]
-    c01 = np.einsum('nij,nj->ni', Tm, xi[ok] - r)
-    bary[ok, :2] = c01
-    bary[ok, 2] = 1.0 - c01.sum(axis=1)
-    nearest = cKDTree(pts).query(xi, workers=kd_workers)[1]
     # bilinear back-interpolation (RegularGridInterpolator 'linear') at the nodes
     ii = np.clip(np.searchsorted(mgrid, pts[:, 0]) - 1, 0, G - 2)
     jj = np.clip(np.searchsorted(mgrid, pts[:, 1]) - 1, 0, G - 2)
@@ -161,11 +163,11 @@ def build_geometry(mtot, mrat, refine=4, kd_workers=-1):
     wy = (pts[:, 1] - mgrid[jj]) / (mgrid[jj + 1] - mgrid[jj])
     back_idx = np.stack([ii * G + jj, ii * G + jj + 1, (ii + 1) * G + jj, (ii + 1) * G + jj + 1], axis=-1)
     back_w = np.stack([(1 - wx) * (1 - wy), (1 - wx) * wy, wx * (1 - wy), wx * wy], axis=-1)
+    T = _transforms(pts, tri.simplices)
     host = dict(
-        pts=pts, nbr_ptr=indptr.astype(np.int32), nbr_idx=indices.astype(np.int32),
+        pts=pts, nbr_ptr=indptr, nbr_idx=indices,
         levels_ptr=lptr.astype(np.int32), levels_idx=lorder.astype(np.int32),
-        simplices=tri.simplices.astype(np.int32), simplex_g=_simplex_g(tri), q_simplex=simp.astype(np.int32),
-        q_bary=bary, q_nearest=nearest.astype(np.int32),
+        simplices=tri.simplices.astype(np.int32), simplex_g=_simplex_g(tri, T), transform=T, mgrid=mgrid,
         back_idx=back_idx.astype(np.int32), back_w=back_w)
     return dict(host=host, M=M, Q=Q, G=G, N=N, n_levels=nlev, max_level_width=max_level_width, n_ovf=n_ovf,
                 mgrid=mgrid)
@@ -199,6 +201,14 @@ class ScatterPlan:
             d = {}
             for k, v in h.items():
                 d[k] = D.to_dev(v, dtype=torch.int32 if v.dtype == np.int32 else torch.float64)
+            # the regular grid's simplices, barycentric coordinates and nearest nodes
+            G2 = self.G * self.G
+            d['q_simplex'] = D.empty(G2, dtype=torch.int32)
+            d['q_bary'] = D.empty((G2, 3))
+            d['q_nearest'] = D.empty(G2, dtype=torch.int32)
+            _lib.call("hd_ct_locate", D.ptr(d['pts']), self.N, D.ptr(d['simplices']), D.ptr(d['transform']),
+                      int(h['simplices'].shape[0]), D.ptr(d['mgrid']), self.G, D.ptr(d['q_simplex']),
+                      D.ptr(d['q_bary']), D.ptr(d['q_nearest']), D.stream_ptr())
             self._dev = d
             self._ready = torch.cuda.Event()
             self._ready.record(cur)

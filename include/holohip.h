@@ -107,6 +107,19 @@ int hd_scatter(double *dens, int M, int Q, int Z, int G, const double *pts, cons
                const int32_t *simplices, const double *simplex_g, const int32_t *q_simplex, const double *q_bary,
                const int32_t *q_nearest, const double *P, const int32_t *back_idx, const double *back_w, double tol,
                int maxiter, void *workspace, size_t ws_bytes, int32_t *iters_out, void *stream);
+/* The scatter geometry's device part, from the triangulation of the N nodes (pts (N,2);
+ * simplices (S,3) and their affine transforms (S,3,2), as scipy.spatial.Delaunay.transform):
+ * for every point xi[i G + j] = (mgrid[i], mgrid[j]) of the regular grid, the simplex holding
+ * it (-1 outside the hull; Qhull's inside test with eps = 100 DBL_EPSILON, the lowest simplex
+ * for a point on a shared edge) with its barycentric coordinates q_bary (G*G,3), and the
+ * nearest node q_nearest (squared distance, lowest index on a tie) -- the lookups of
+ * CloughTocher2DInterpolator and NearestNDInterpolator in add_scatter_to_masses
+ * (sam.py:1365-1376). */
+int hd_ct_locate(const double *pts, int N, const int32_t *simplices, const double *transform, int S,
+                 const double *mgrid, int G, int32_t *q_simplex, double *q_bary, int32_t *q_nearest, void *stream);
+/* Host: the Gauss-Seidel level of every node (one more than the highest level of its
+ * lower-indexed neighbours, 0 without any), from the vertex adjacency (nbr_ptr (N+1), nbr_idx). */
+int hd_ct_levels(const int32_t *nbr_ptr, const int32_t *nbr_idx, int N, int32_t *level);
 /* C (m,n) = A (m,k) B (k,n), row-major FP64 (the scatter's convolution GEMM;
  * the reference evaluates it with numpy.einsum, utils.py:455-461). */
 int hd_gemm_f64(const double *A, const double *B, double *C, int m, int n, int k, void *stream);
