@@ -200,6 +200,44 @@ def test_ct_gradients_bit_exact_vs_scipy(gpu):
         assert it[z] > 0
 
 
+@pytest.mark.parametrize("shape", [(17, 15), (61, 51), (91, 81)])
+def test_ct_locate_matches_scipy(gpu, shape):
+    """hd_ct_locate on the scatter's regular grid against scipy: every point scipy's
+    find_simplex places is placed in a simplex that holds it (the same one, or on a shared edge a
+    neighbour: both barycentric tests pass), with barycentric coordinates to 1e-12; points outside
+    the hull get -1 as in scipy; the nearest node is cKDTree's (or one at the same distance)."""
+    from scipy.spatial import Delaunay, cKDTree
+    from holodeck_amd import scatter
+    from holodeck_amd import device as D
+    from holodeck_amd.constants import MSOL
+    M, Q = shape
+    print(f"[ct locate {shape}]", flush=True)
+    mtot = np.logspace(np.log10(1e4 * MSOL), np.log10(1e12 * MSOL), M)
+    mrat = np.logspace(-3, 0, Q)
+    plan = scatter.ScatterPlan(mtot, mrat)
+    d = plan.device()
+    simp, bary, near = D.to_host(d['q_simplex']), D.to_host(d['q_bary']), D.to_host(d['q_nearest'])
+    pts, mgrid, G = plan.host['pts'], plan.mgrid, plan.G
+    X, Y = np.meshgrid(mgrid, mgrid, indexing='ij')
+    xi = np.stack([X.ravel(), Y.ravel()], axis=-1)
+    tri = Delaunay(pts)
+    want = tri.find_simplex(xi)
+    assert np.array_equal(simp < 0, want < 0), (np.sum(simp < 0), np.sum(want < 0))
+    ins = want >= 0
+    diff = ins & (simp != want)
+    T = tri.transform
+    for k in np.flatnonzero(diff):   # a different simplex only where both hold the point
+        c = T[simp[k], :2] @ (xi[k] - T[simp[k], 2])
+        c = np.append(c, 1 - c.sum())
+        assert np.all(c >= -1e-13) and np.all(c <= 1 + 1e-13), (k, c)
+    same = ins & ~diff
+    c01 = np.einsum('nij,nj->ni', T[want[same], :2, :], xi[same] - T[want[same], 2, :])
+    assert np.allclose(bary[same, :2], c01, rtol=0, atol=1e-12)
+    dist, nn = cKDTree(pts).query(xi)
+    dgot = np.hypot(*(xi - pts[near]).T)
+    assert np.all((near == nn) | (np.abs(dgot - dist) <= 1e-15 * np.maximum(dist, 1e-300)))
+
+
 def test_stageA_full_config1_with_scatter(gpu, oracle_lib):
     """number and h2fdf at the full config-1 workload (61x51x41, 20 frequencies, GW-only)."""
     print("[stageA full config1]", flush=True)

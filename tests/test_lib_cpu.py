@@ -104,6 +104,39 @@ def test_density_param_layout_matches_kernel_enum():
     assert p2[pos['DP_GMR_MREF']] == sams.GMR_Illustris()._mref
 
 
+@pytest.mark.parametrize("shape", [(17, 15), (41, 81), (91, 81)])
+def test_level_schedule_and_transforms_on_the_host(shape):
+    """The scatter geometry's host part: hd_ct_levels gives every node one more than the highest
+    level of its lower-indexed neighbours (a direct Python loop over scipy's adjacency), so each
+    level only reads finished lower neighbours; the closed-form simplex transforms agree with
+    scipy.spatial.Delaunay.transform to 1e-12 relative."""
+    from scipy.spatial import Delaunay
+    from holodeck_amd import scatter
+    from holodeck_amd.constants import MSOL
+    M, Q = shape
+    mtot = np.logspace(np.log10(1e4 * MSOL), np.log10(1e12 * MSOL), M)
+    mrat = np.logspace(-3, 0, Q)
+    g = scatter.build_geometry(mtot, mrat)
+    h = g['host']
+    ip, ix, N = h['nbr_ptr'], h['nbr_idx'], g['N']
+    want = np.zeros(N, dtype=np.int64)
+    for p in range(N):
+        nb = ix[ip[p]:ip[p + 1]]
+        lo = nb[nb < p]
+        want[p] = want[lo].max() + 1 if lo.size else 0
+    got = np.zeros(N, dtype=np.int64)
+    lp, li = h['levels_ptr'], h['levels_idx']
+    for L in range(lp.size - 1):
+        got[li[lp[L]:lp[L + 1]]] = L
+    assert np.array_equal(got, want)
+    for L in range(lp.size - 1):   # a level's nodes have all lower neighbours in earlier levels
+        for p in li[lp[L]:lp[L + 1]]:
+            nb = ix[ip[p]:ip[p + 1]]
+            assert np.all(got[nb[nb < p]] < L)
+    T = Delaunay(h['pts']).transform
+    assert np.allclose(h['transform'], T, rtol=1e-12, atol=1e-12 * np.abs(T).max())
+
+
 def test_poisson_window_sizing_on_the_host():
     """The CDF tables of the realization kernels, sized on the host by the same function: the
     support drops less than 2^-56 below its first count kb (scipy's Poisson CDF), the table covers
