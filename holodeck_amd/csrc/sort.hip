@@ -6,17 +6,19 @@
 //
 // LSD radix sort over the 64-bit order-preserving image of the key, 8 bits per
 // pass.  Each pass: per-tile digit histogram, one exclusive scan of the
-// (digit-major, tile-minor) counts, and a stable scatter in which one
-// wavefront walks its tile in chunks of 64 and ranks equal digits with
-// ballots.  Passes whose digit is the same for every key are skipped on the
+// (digit-major, tile-minor) counts, and a stable scatter in which each of a
+// tile's four wavefronts walks a quarter of it in chunks of 64 and ranks equal
+// digits with ballots.  Passes whose digit is the same for every key are skipped on the
 // device (the scatter kernel copies).
 #include "common.hpp"
 #include "../../include/holohip.h"
 
 namespace hd {
 
-constexpr int SORT_TILE = 4096;     // keys per tile (one wavefront)
+constexpr int SORT_TILE = 4096;     // keys per tile
+constexpr int SORT_WAVES = 4;       // waves per tile: each ranks a contiguous quarter
 constexpr int SORT_RADIX = 256;
+constexpr int SORT_SCAN_THREADS = 1024;
 
 __device__ __forceinline__ uint64_t desc_key(double v) {
     if (v == 0.0) v = 0.0;   // -0 -> +0
@@ -33,95 +35,135 @@ __global__ void k_sort_init(const double* __restrict__ key, int64_t B, int64_t s
     v_out[i] = i;
 }
 
-__global__ void __launch_bounds__(64) k_sort_hist(const uint64_t* __restrict__ keys, int64_t B, int shift,
-                                                  uint32_t* __restrict__ hist, int ntiles) {
+// digit histogram of one tile (256 threads, LDS atomics), stored digit-major
+__global__ void __launch_bounds__(256) k_sort_hist(const uint64_t* __restrict__ keys, int64_t B, int shift,
+                                                   uint32_t* __restrict__ hist, int ntiles) {
     __shared__ uint32_t h[SORT_RADIX];
-    for (int d = threadIdx.x; d < SORT_RADIX; d += 64) h[d] = 0;
+    h[threadIdx.x] = 0;
     __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-    int64_t end = min(base + SORT_TILE, B);
-    for (int64_t i = base + threadIdx.x; i < end; i += 64) {
-        uint32_t d = (uint32_t)(keys[i] >> shift) & 0xff;
-        atomicAdd(&h[d], 1u);
-    }
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+    const int64_t end = min(base + SORT_TILE, B);
+    for (int64_t i = base + threadIdx.x; i < end; i += 256) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xff], 1u);
     __syncthreads();
-    for (int d = threadIdx.x; d < SORT_RADIX; d += 64) hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of n (<= a few 1e5) counts by one workgroup of 1024 threads;
-// also flags whether a single digit holds every key.
-__global__ void __launch_bounds__(1024) k_sort_scan(uint32_t* __restrict__ data, int64_t n, int ntiles, int64_t B,
-                                                    int* __restrict__ trivial) {
-    __shared__ uint32_t s_part[1024];
+// exclusive scan of n counts (16-byte aligned) by one workgroup: each thread sums a contiguous
+// segment (four counts per load), the segment sums are scanned in LDS, and each thread rewrites
+// its segment; also flags whether a single digit holds every key
+__global__ void __launch_bounds__(SORT_SCAN_THREADS) k_sort_scan(uint32_t* __restrict__ data, int64_t n, int ntiles,
+                                                                 int64_t B, int* __restrict__ trivial) {
+    __shared__ uint32_t s_part[SORT_SCAN_THREADS];
     __shared__ int s_triv;
     const int tid = threadIdx.x;
-    int64_t per = (n + 1023) / 1024;
-    int64_t lo = tid * per, hi = min(lo + per, n);
+    const int64_t per = ((n + SORT_SCAN_THREADS - 1) / SORT_SCAN_THREADS + 3) & ~(int64_t)3;
+    const int64_t lo = min((int64_t)tid * per, n), hi = min(lo + per, n);
+    const int64_t hi4 = lo + ((hi - lo) & ~(int64_t)3);
     uint32_t acc = 0;
-    for (int64_t i = lo; i < hi; i++) acc += data[i];
+#pragma unroll 4
+    for (int64_t i = lo; i < hi4; i += 4) {
+        const uint4 v = *(const uint4*)(data + i);
+        acc += v.x + v.y + v.z + v.w;
+    }
+    for (int64_t i = hi4; i < hi; i++) acc += data[i];
     s_part[tid] = acc;
     if (tid == 0) s_triv = 0;
     __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partials
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = (tid >= off) ? s_part[tid - off] : 0u;
+    for (int off = 1; off < SORT_SCAN_THREADS; off <<= 1) {   // Hillis-Steele over the segment sums
+        const uint32_t v = (tid >= off) ? s_part[tid - off] : 0u;
         __syncthreads();
         s_part[tid] += v;
         __syncthreads();
     }
     uint32_t run = (tid == 0) ? 0u : s_part[tid - 1];
-    for (int64_t i = lo; i < hi; i++) {
-        uint32_t v = data[i];
+#pragma unroll 4
+    for (int64_t i = lo; i < hi4; i += 4) {
+        uint4 v = *(uint4*)(data + i);
+        uint4 o;
+        o.x = run;
+        run += v.x;
+        o.y = run;
+        run += v.y;
+        o.z = run;
+        run += v.z;
+        o.w = run;
+        run += v.w;
+        *(uint4*)(data + i) = o;
+    }
+    for (int64_t i = hi4; i < hi; i++) {
+        const uint32_t v = data[i];
         data[i] = run;
         run += v;
     }
     __syncthreads();
     // trivial pass: some digit d has offsets [d*ntiles] == 0 and [(d+1)*ntiles] == B
-    for (int d = tid; d < SORT_RADIX; d += 1024) {
-        uint32_t start = data[(int64_t)d * ntiles];
-        uint32_t stop = (d + 1 < SORT_RADIX) ? data[(int64_t)(d + 1) * ntiles] : (uint32_t)B;
+    for (int d = tid; d < SORT_RADIX; d += SORT_SCAN_THREADS) {
+        const uint32_t start = data[(int64_t)d * ntiles];
+        const uint32_t stop = (d + 1 < SORT_RADIX) ? data[(int64_t)(d + 1) * ntiles] : (uint32_t)B;
         if (start == 0 && stop == (uint32_t)B) s_triv = 1;
     }
     __syncthreads();
     if (tid == 0) *trivial = s_triv;
 }
 
-__global__ void __launch_bounds__(64) k_sort_scatter(const uint64_t* __restrict__ kin, const int64_t* __restrict__ vin,
-                                                     uint64_t* __restrict__ kout, int64_t* __restrict__ vout, int64_t B,
-                                                     int shift, const uint32_t* __restrict__ offs, int ntiles,
-                                                     const int* __restrict__ trivial) {
-    __shared__ uint32_t run[SORT_RADIX];
-    const int lane = threadIdx.x;
-    int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-    int64_t end = min(base + SORT_TILE, B);
+// Stable scatter of one tile by four waves, each ranking a contiguous quarter in chunks of 64
+// keys with ballots; a wave's digit offsets are the tile's plus the counts of the quarters
+// before it.  Within a wave the running offsets are read and then advanced in program order
+// (LDS operations of a wave complete in order), so no workgroup barrier is taken per chunk.
+__global__ void __launch_bounds__(64 * SORT_WAVES) k_sort_scatter(const uint64_t* __restrict__ kin,
+                                                                  const int64_t* __restrict__ vin,
+                                                                  uint64_t* __restrict__ kout,
+                                                                  int64_t* __restrict__ vout, int64_t B, int shift,
+                                                                  const uint32_t* __restrict__ offs, int ntiles,
+                                                                  const int* __restrict__ trivial) {
+    __shared__ uint32_t s_h[SORT_WAVES][SORT_RADIX];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+    const int64_t end = min(base + SORT_TILE, B);
     if (*trivial) {
-        for (int64_t i = base + lane; i < end; i += 64) {
+        for (int64_t i = base + tid; i < end; i += 64 * SORT_WAVES) {
             kout[i] = kin[i];
             vout[i] = vin[i];
         }
         return;
     }
-    for (int d = lane; d < SORT_RADIX; d += 64) run[d] = offs[(int64_t)d * ntiles + blockIdx.x];
+    constexpr int QUARTER = SORT_TILE / SORT_WAVES;
+    const int64_t wb = base + (int64_t)w * QUARTER, we = min(wb + QUARTER, end);
+    for (int d = lane; d < SORT_RADIX; d += 64) s_h[w][d] = 0;
     __syncthreads();
+    for (int64_t i = wb + lane; i < we; i += 64) atomicAdd(&s_h[w][(uint32_t)(kin[i] >> shift) & 0xff], 1u);
+    __syncthreads();
+    {   // thread d: the offsets of digit d for the four quarters
+        const int d = tid;
+        uint32_t run = offs[(int64_t)d * ntiles + blockIdx.x];
+#pragma unroll
+        for (int q = 0; q < SORT_WAVES; q++) {
+            const uint32_t c = s_h[q][d];
+            s_h[q][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    uint32_t* run = s_h[w];
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int64_t c = base; c < end; c += 64) {
-        int64_t i = c + lane;
-        bool valid = i < end;
-        uint64_t k = valid ? kin[i] : 0;
-        uint32_t d = (uint32_t)(k >> shift) & 0xff;
+    for (int64_t c = wb; c < we; c += 64) {
+        const int64_t i = c + lane;
+        const bool valid = i < we;
+        const uint64_t k = valid ? kin[i] : 0;
+        const uint32_t d = (uint32_t)(k >> shift) & 0xff;
         uint64_t eq = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; bit++) {
-            uint64_t b = __ballot(valid && ((d >> bit) & 1));
+            const uint64_t b = __ballot(valid && ((d >> bit) & 1));
             eq &= ((d >> bit) & 1) ? b : ~b;
         }
-        uint32_t rank = (uint32_t)__popcll(eq & lt_mask);
-        uint32_t cnt = (uint32_t)__popcll(eq);
+        const uint32_t rank = (uint32_t)__popcll(eq & lt_mask);
+        const uint32_t cnt = (uint32_t)__popcll(eq);
         uint32_t pos = 0;
         if (valid) pos = run[d] + rank;
-        __syncthreads();   // all lanes have read run[] for this chunk
+        asm volatile("" ::: "memory");   // every lane's read of run[] before any advance
         if (valid && rank == cnt - 1) run[d] = pos + 1;
-        __syncthreads();
+        asm volatile("" ::: "memory");
         if (valid) {
             kout[pos] = k;
             vout[pos] = vin[i];
@@ -138,7 +180,7 @@ extern "C" size_t hd_order_workspace_size(int64_t B) {
     size_t s = 0;
     s += 2 * (size_t)B * sizeof(uint64_t);   // keys ping-pong (2nd buffer)
     s += (size_t)B * sizeof(int64_t);        // values ping buffer
-    s += (size_t)SORT_RADIX * ntiles * sizeof(uint32_t);
+    s += (size_t)SORT_RADIX * ntiles * sizeof(uint32_t) + 256;   // counts (256-byte aligned)
     s += 256;                                // trivial flag + alignment
     return s + 1024;
 }
@@ -157,6 +199,7 @@ extern "C" int hd_order_desc(const double* key, int64_t B, int64_t stride, int64
     uint64_t* k0 = (uint64_t*)p;            p += (size_t)B * 8;
     uint64_t* k1 = (uint64_t*)p;            p += (size_t)B * 8;
     int64_t* v1 = (int64_t*)p;              p += (size_t)B * 8;
+    p = (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
     uint32_t* hist = (uint32_t*)p;          p += (size_t)SORT_RADIX * ntiles * 4;
     int* triv = (int*)(((uintptr_t)p + 15) & ~(uintptr_t)15);
     int64_t* v0 = order;   // values ping-pong through the output buffer; 8 passes end in v0
@@ -168,10 +211,11 @@ extern "C" int hd_order_desc(const double* key, int64_t B, int64_t stride, int64
     int64_t* vin = v0; int64_t* vout = v1;
     for (int pass = 0; pass < 8; pass++) {
         int shift = pass * 8;
-        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(64), 0, st, kin, B, shift, hist, ntiles);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, hist, (int64_t)SORT_RADIX * ntiles, ntiles, B, triv);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(64), 0, st, kin, vin, kout, vout, B, shift, hist, ntiles,
-                           triv);
+        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(256), 0, st, kin, B, shift, hist, ntiles);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(SORT_SCAN_THREADS), 0, st, hist, (int64_t)SORT_RADIX * ntiles,
+                           ntiles, B, triv);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(64 * SORT_WAVES), 0, st, kin, vin, kout, vout, B, shift,
+                           hist, ntiles, triv);
         rc = check_launch("k_sort pass");
         if (rc) return rc;
         uint64_t* tk = kin; kin = kout; kout = tk;
