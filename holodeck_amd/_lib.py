@@ -39,9 +39,9 @@ SIGNATURES = {
                            _P, _P, _P, _I32, _P, _P, _P]),
     "hd_integrate_3dx1d": (_I32, [_P, _I32, _P, _I32, _P, _I32, _P, _I32, _P, _P, _P]),
     "hd_h2fdf_redz": (_I32, [_P, _I32, _P, _I32, _I32, _P, _P, _I32, _P, _D, _D, _D, _P, _P, _I32, _P, _P]),
-    "hd_scatter_workspace_size": (_SZ, [_I32, _I32, _I32, _I32, _I32]),
-    "hd_scatter": (_I32, [_P, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P,
-                          _P, _P, _D, _I32, _P, _SZ, _P, _P]),
+    "hd_scatter_workspace_size": (_SZ, [_I32, _I32, _I32, _I32, _I32, _I32, _I32]),
+    "hd_scatter": (_I32, [_P, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P,
+                          _P, _P, _P, _P, _P, _D, _I32, _P, _SZ, _P, _P]),
     "hd_gemm_f64": (_I32, [_P, _P, _P, _I32, _I32, _I32, _P]),
     "hd_ct_locate": (_I32, [_P, _I32, _P, _P, _I32, _P, _I32, _P, _P, _P, _P]),
     "hd_ct_levels": (_I32, [_P, _P, _I32, _P]),

@@ -368,6 +368,225 @@ __global__ void __launch_bounds__(CT_MAX_THREADS) k_ct_gradients(
     if (tid == 0) iters_out[z] = iters;
 }
 
+// The sweeps pipelined: sweep s + 1 starts on its early levels while sweep s is still on
+// its late ones.  (Vertex i, sweep s) runs at step l(i) + s P, P one more than the largest
+// level difference across an edge: a vertex then reads its lower neighbours after their
+// update of the same sweep and its higher neighbours after their update of the previous
+// sweep and before their update of this one -- exactly the values the sequential sweep
+// reads, so the result is bit for bit the same.  A step runs the levels l = t - s P of the
+// sweeps in flight (at most ceil(levels / P)), one group of four lanes per vertex (two
+// entries each; otherwise as in k_ct_gradients).  Each update also writes its sweep's value to a ring of K sweeps in
+// global memory (y_hist) and raises the sweep's largest relative change (an LDS atomic max
+// on the non-negative doubles' bits); when a sweep's last level is done its change is
+// compared with tol, and on convergence (or at maxiter) the gradients are that sweep's ring
+// slot: the later sweeps in flight have overwritten some of the solution in LDS, but they
+// write other ring slots, and slot s is reused only by sweep s + K, which starts after the
+// check (K P > levels + P; the change slots likewise).
+// Step tables per residue r = t mod P (levels r, r + P, ...: their first slots, prefix
+// widths, and the map from a group to its level and place in it) are built in LDS at the start.  The workgroup holds four lanes for every vertex of
+// the widest step; the host checks that and the LDS before choosing this kernel.
+constexpr int CT_MAX_K = 16;
+constexpr int CT_PIPE_LANES = 4;   // lanes per vertex in the pipelined sweeps (two entries each)
+
+__global__ void __launch_bounds__(CT_MAX_THREADS) k_ct_gradients_pipe(
+    int N, int Z, const int4* __restrict__ slot_i, const double* __restrict__ slot_v, const int32_t* __restrict__ pe_q,
+    const double4* __restrict__ pe_geo, const double* __restrict__ A, int64_t ecap, const int32_t* __restrict__ lptr,
+    int n_levels, int P, int K, int W, double tol, int maxiter, double* __restrict__ y_hist,
+    double* __restrict__ grad_g, int32_t* __restrict__ iters_out) {
+    extern __shared__ double4 s_dyn[];
+    __shared__ unsigned long long s_err[CT_MAX_K];
+    __shared__ int s_bad;
+    const int z = blockIdx.x;
+    const int nthr = blockDim.x;
+    const int tid = threadIdx.x;
+    const int j = tid / CT_PIPE_LANES, k = tid % CT_PIPE_LANES;   // lane k holds entries k and k + 4
+    const double* Az = A + (int64_t)z * ecap;
+    double2* hz = (double2*)(y_hist + (int64_t)z * K * N * 2);
+    const int KM = (n_levels + P - 1) / P;   // levels per residue, at most
+    // LDS: [solution N double2] [terms 8 per group double2] [level slot pointers n+1] [per residue:
+    // level count P, first slots P KM, prefix widths P (KM + 1), step map P W (16-bit)]
+    double2* y = (double2*)s_dyn;
+    double2* T = y + N;   // the terms of group g's entries at T[g CT_SLOT .. g CT_SLOT + 7]
+    int32_t* s_lptr = (int32_t*)(T + (nthr / CT_PIPE_LANES) * CT_SLOT);
+    int32_t* s_cnt = s_lptr + n_levels + 1;
+    int32_t* s_start = s_cnt + P;
+    int32_t* s_pre = s_start + P * KM;
+    uint16_t* s_map = (uint16_t*)(s_pre + P * (KM + 1));   // (level k of the residue) << 12 | index in the level
+    for (int i = tid; i < N; i += nthr) y[i] = make_double2(0.0, 0.0);
+    for (int i = tid; i <= n_levels; i += nthr) s_lptr[i] = lptr[i];
+    if (tid < CT_MAX_K) s_err[tid] = 0ull;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    for (int r = tid; r < P; r += nthr) {
+        int c = 0, acc = 0;
+        for (int l = r; l < n_levels; l += P, c++) {
+            s_start[r * KM + c] = s_lptr[l];
+            s_pre[r * (KM + 1) + c] = acc;
+            acc += s_lptr[l + 1] - s_lptr[l];
+        }
+        s_pre[r * (KM + 1) + c] = acc;
+        s_cnt[r] = c;
+        if (acc * CT_PIPE_LANES > nthr || acc > W) s_bad = 1;   // a step wider than the workgroup: refuse
+        else
+            for (int l = r, kk = 0; l < n_levels; l += P, kk++)
+                for (int i = 0; i < s_lptr[l + 1] - s_lptr[l]; i++)
+                    s_map[r * W + s_pre[r * (KM + 1) + kk] + i] = (uint16_t)((kk << 12) | i);
+    }
+    __syncthreads();
+    if (s_bad) {
+        if (tid == 0) {
+            atomicOr(iters_out + Z, 1 << 21);
+            iters_out[z] = 0;
+        }
+        return;
+    }
+    // the vertex of this lane's group at step t = q P + r (false: none) and its sweep; the
+    // prefix widths of the residue are read together and counted, no dependent search
+    auto locate = [&](int q, int r, int& slot, int& sweep) -> bool {
+        if (q >= KM - 1 && q < maxiter) {   // every level of every residue in flight: the map directly
+            if (j >= s_pre[r * (KM + 1) + s_cnt[r]]) return false;
+            const int e = s_map[r * W + j];
+            slot = s_start[r * KM + (e >> 12)] + (e & 4095);
+            sweep = q - (e >> 12);
+            return true;
+        }
+        const int cnt = s_cnt[r];
+        const int khi = min(cnt - 1, q);                 // levels r + k P <= t
+        const int klo = max(0, q - maxiter + 1);         // sweeps (t - l) / P < maxiter
+        if (klo > khi) return false;
+        const int32_t* pre = s_pre + r * (KM + 1);
+        const int g = j + pre[klo];
+        if (g >= pre[khi + 1]) return false;
+        const int e = s_map[r * W + g];
+        slot = s_start[r * KM + (e >> 12)] + (e & 4095);
+        sweep = q - (e >> 12);
+        return true;
+    };
+    const uint32_t elast = (uint32_t)N * CT_SLOT - 1;
+    int cq, cq2;
+    double4 cg, cg2;
+    double ca, ca2;
+    int4 vi;
+    double vQ0, vQ1, vQ3, vdet, vrdet;
+    auto load_entry = [&](int slot) {
+        const uint32_t e = min((uint32_t)(slot * CT_SLOT + k), elast);
+        const uint32_t e2 = min(e + CT_PIPE_LANES, elast);
+        cq = *(const int32_t*)((const char*)pe_q + (e << 2));
+        cq2 = *(const int32_t*)((const char*)pe_q + (e2 << 2));
+        cg = *(const double4*)((const char*)pe_geo + (e << 5));
+        cg2 = *(const double4*)((const char*)pe_geo + (e2 << 5));
+        ca = *(const double*)((const char*)Az + (e << 3));
+        ca2 = *(const double*)((const char*)Az + (e2 << 3));
+    };
+    auto load_vert = [&](int slot) {
+        const uint32_t t = min((uint32_t)slot, (uint32_t)(N - 1));
+        vi = *(const int4*)((const char*)slot_i + (t << 4));
+        const double* sv = (const double*)((const char*)slot_v + t * 48u);
+        vQ0 = sv[0];
+        vQ1 = sv[1];
+        vQ3 = sv[2];
+        vdet = sv[3];
+        vrdet = sv[4];
+    };
+    // steps t (current) and t + 1 located; t + 2 is located during step t, after its updates
+    int slot = 0, sweep = 0, slot1 = 0, sweep1 = 0;
+    int q2 = 2 / P, r2 = 2 % P;   // t + 2 = q2 P + r2
+    bool act = locate(0, 0, slot, sweep);
+    bool act1 = locate(1 / P, 1 % P, slot1, sweep1);
+    load_entry(act ? slot : 0);
+    load_vert(act ? slot : 0);
+    int iters = 0, fin = 0;
+    for (int t = 0;; t++) {
+        // the entry's terms
+        if (act) {
+            const double2 yq = y[cq >= 0 ? cq : 0], yq2 = y[cq2 >= 0 ? cq2 : 0];
+            double2 tt = ct_terms(ca, cg, yq), tt2 = ct_terms(ca2, cg2, yq2);
+            if (cq < 0) tt = make_double2(-0.0, -0.0);
+            if (cq2 < 0) tt2 = make_double2(-0.0, -0.0);
+            T[j * CT_SLOT + k] = tt;
+            T[j * CT_SLOT + k + CT_PIPE_LANES] = tt2;
+        }
+        load_entry(act1 ? slot1 : 0);   // the next step's entry
+        asm volatile("" ::: "memory");
+        // the group's first lane: in-order sums, overflow entries, solve, change, ring
+        if (act && k == 0) {
+            double2 tk[CT_SLOT];
+#pragma unroll
+            for (int i = 0; i < CT_SLOT; i++) tk[i] = T[j * CT_SLOT + i];
+            const double2 yo = y[vi.x];
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int i = 0; i < CT_SLOT; i++) {
+                s0 += tk[i].x;
+                s1 += tk[i].y;
+            }
+            for (int64_t e = (int64_t)N * CT_SLOT + vi.y; e < (int64_t)N * CT_SLOT + vi.z; e++) {   // rare
+                const int q = pe_q[e];
+                const double2 tt = ct_terms(Az[e], pe_geo[e], y[q]);
+                s0 += tt.x;
+                s1 += tt.y;
+            }
+            const double Q2 = vQ1;
+            const double n0 = vQ3 * s0 - vQ1 * s1;
+            const double n1 = -Q2 * s0 + vQ0 * s1;
+            const double q0 = n0 * vrdet, q1 = n1 * vrdet;
+            double r0 = ct_div_fast(n0, vdet, vrdet, q0);
+            double r1 = ct_div_fast(n1, vdet, vrdet, q1);
+            if (!((vdet > 0.0) & ct_div_ok(n0, q0) & ct_div_ok(n1, q1))) {   // rare: exactly rounded either way
+                r0 = n0 / vdet;
+                r1 = n1 / vdet;
+            }
+            const double2 yn = make_double2(-r0, -r1);
+            y[vi.x] = yn;
+            hz[(int64_t)(sweep % K) * N + vi.x] = yn;
+            // scipy's change: max(|y_old + r|) / max(1, max|r|)
+            const double a0 = fabs(yo.x + r0), a1 = fabs(yo.y + r1);
+            const double m0 = fabs(r0), m1 = fabs(r1);
+            const double num = (a0 > a1) ? a0 : a1;
+            const double mr = (m0 > m1) ? m0 : m1;
+            const double den = (1.0 > mr) ? 1.0 : mr;
+            const double ch = num / den;
+            atomicMax(s_err + sweep % K, (unsigned long long)__double_as_longlong(ch));
+        }
+        load_vert(act1 ? slot1 : 0);   // the next step's vertex
+        int slot2 = 0, sweep2 = 0;
+        const bool act2 = locate(q2, r2, slot2, sweep2);
+        if (++r2 == P) {
+            r2 = 0;
+            q2++;
+        }
+        ct_barrier_lds();
+        act = act1;
+        slot = slot1;
+        sweep = sweep1;
+        act1 = act2;
+        slot1 = slot2;
+        sweep1 = sweep2;
+        // a sweep whose last level was this step: its convergence test
+        const int tl = t - (n_levels - 1);
+        if (tl >= 0 && tl % P == 0) {
+            const int sc = tl / P;
+            const double m = __longlong_as_double((long long)s_err[sc % K]);
+            if (m < tol) {
+                iters = sc + 1;
+                fin = sc;
+                break;
+            }
+            if (sc + 1 >= maxiter) {
+                fin = sc;
+                break;
+            }
+            __syncthreads();   // every thread has read the slot before it is cleared
+            if (tid == 0) s_err[sc % K] = 0ull;
+        }
+    }
+    __syncthreads();   // the ring writes of every thread are complete
+    const double2* src = hz + (int64_t)(fin % K) * N;
+    double2* dst = (double2*)(grad_g + (int64_t)z * N * 2);
+    for (int i = tid; i < N; i += nthr) dst[i] = src[i];
+    if (tid == 0) iters_out[z] = iters;
+}
+
 // Clough-Tocher value at barycentric b in triangle s (vertex order of `simp`).
 __device__ double ct_value(const double* __restrict__ pts, const int32_t* __restrict__ simp, const double* g3,
                            const double* b, const double* f, const double* df) {
@@ -646,29 +865,38 @@ extern "C" int hd_scatter_weights_T(const double* dm, int G, double* wT, void* s
     return check_launch("k_rolled_weights_T");
 }
 
-extern "C" size_t hd_scatter_workspace_size(int M, int Q, int Z, int G, int n_ovf) {
+// sweeps kept in the pipelined sweep's ring (0: the sweeps run one after another)
+static int ct_ring(int n_levels, int sweep_period) {
+    if (sweep_period < 1 || n_levels < 1) return 0;
+    const int K = (n_levels + sweep_period) / sweep_period + 2;
+    return K <= CT_MAX_K ? K : 0;
+}
+
+extern "C" size_t hd_scatter_workspace_size(int M, int Q, int Z, int G, int n_ovf, int n_levels, int sweep_period) {
     size_t N = (size_t)M * Q;
     const size_t ecap = CT_SLOT * N + (size_t)(n_ovf > 0 ? n_ovf : 0);
+    const size_t K = (size_t)ct_ring(n_levels, sweep_period);
     return align256(N * Z * 2 * 8) + 2 * align256((size_t)G * G * Z * 8) + align256(N * Z * 8) +   // grad, D, Y, densT
            align256((N + 1) * 4) + align256(N * 16) + align256(N * 6 * 8) +   // slot overflow offsets and records
            align256(ecap * 4) + align256(ecap * 32) +                         // adjacency entries
            align256(N * Z * 2 * 8) +                                          // solution when it exceeds the LDS
-           align256((size_t)Z * ecap * 8) + 256;                              // 6 (f1 - f2) per slice and entry
+           align256((size_t)Z * ecap * 8) +                                   // 6 (f1 - f2) per slice and entry
+           align256(K * N * Z * 2 * 8) + 256;                                 // the pipelined sweeps' ring
 }
 
 extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double* pts, const int32_t* nbr_ptr,
                           const int32_t* nbr_idx, const int32_t* levels_ptr, const int32_t* levels_idx, int n_levels,
-                          int max_level_width, int n_ovf,
+                          int max_level_width, int n_ovf, int sweep_period, int max_step_width,
                           const int32_t* simplices, const double* simplex_g, const int32_t* q_simplex,
                           const double* q_bary, const int32_t* q_nearest, const double* P, const int32_t* back_idx,
                           const double* back_w, double tol, int maxiter, void* workspace, size_t ws_bytes,
                           int32_t* iters_out, void* stream) {
-    if (M < 2 || Q < 2 || Z < 1 || G < 2 || n_levels < 1 || max_level_width < 1 || n_ovf < 0 || !dens || !pts ||
+    if (M < 2 || Q < 2 || Z < 1 || G < 2 || n_levels < 1 || max_level_width < 1 || n_ovf < 0 || maxiter < 1 || !dens || !pts ||
         !nbr_ptr || !nbr_idx || !levels_ptr || !levels_idx || !simplices || !simplex_g || !q_simplex || !q_bary ||
         !q_nearest || !P || !back_idx || !back_w || !iters_out)
         return set_error(HD_ERR_ARG, "hd_scatter: bad arguments");
     if ((int64_t)G * G > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_scatter: G too large");
-    size_t need = hd_scatter_workspace_size(M, Q, Z, G, n_ovf);
+    size_t need = hd_scatter_workspace_size(M, Q, Z, G, n_ovf, n_levels, sweep_period);
     if (!workspace || ws_bytes < need) return set_error(HD_ERR_SIZE, "hd_scatter: workspace too small");
     hipStream_t st = (hipStream_t)stream;
     const int N = M * Q;
@@ -684,7 +912,8 @@ extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double
     int32_t* pe_q = (int32_t*)p;      p += align256((size_t)ecap * 4);
     double4* pe_geo = (double4*)p;    p += align256((size_t)ecap * 32);
     double* ybuf = (double*)p;        p += align256((size_t)N * Z * 2 * 8);
-    double* fdiff = (double*)p;
+    double* fdiff = (double*)p;       p += align256((size_t)Z * ecap * 8);
+    double* y_hist = (double*)p;
     int32_t* bad = iters_out + Z;   // iters_out has Z + 1 entries
 
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, bad, 1);
@@ -702,6 +931,31 @@ extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double
     int max_lds = 0;
     if (he == hipSuccess) he = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (he != hipSuccess) return set_error(HD_ERR_HIP, "hd_scatter: device query: %s", hipGetErrorString(he));
+    // the pipelined sweeps when the ring, the step tables and CT_SLOT lanes per vertex of the
+    // widest step fit
+    const int K = ct_ring(n_levels, sweep_period);
+    if (K > 0 && max_step_width > 0) {
+        const int nthr_p = (int)(((int64_t)max_step_width * CT_PIPE_LANES + 63) / 64 * 64);
+        const int KM = (n_levels + sweep_period - 1) / sweep_period;
+        const size_t lds_p = (size_t)N * 16 + (size_t)(nthr_p / CT_PIPE_LANES) * CT_SLOT * 16 +
+                             (size_t)(n_levels + 1) * 4 + (size_t)sweep_period * (1 + KM + KM + 1) * 4 +
+                             (size_t)sweep_period * max_step_width * 2 + 256;
+        if (nthr_p <= CT_MAX_THREADS && max_level_width < 4096 && lds_p <= (size_t)max_lds) {
+            if (lds_p > 65536) {
+                he = hipFuncSetAttribute((const void*)k_ct_gradients_pipe, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)lds_p);
+                if (he != hipSuccess)
+                    return set_error(HD_ERR_HIP, "hd_scatter: LDS attribute: %s", hipGetErrorString(he));
+            }
+            hipLaunchKernelGGL(k_ct_gradients_pipe, dim3(Z), dim3(nthr_p), lds_p, st, N, Z, slot_i, slot_v, pe_q,
+                               pe_geo, fdiff, ecap, levels_ptr, n_levels, sweep_period, K, max_step_width, tol, maxiter,
+                               y_hist, grad, iters_out);
+            int rcp = check_launch("k_ct_gradients_pipe");
+            if (rcp) return rcp;
+            goto evaluate;
+        }
+    }
+    {
     // CT_SLOT lanes per vertex of the widest level (the kernel refuses a level wider than its workgroup)
     const int nthr = (int)(((int64_t)max_level_width * CT_SLOT + 63) / 64 * 64);
     if ((uint64_t)ecap * 32 >= (1ull << 32))   // the sweep addresses its arrays with 32-bit byte offsets
@@ -727,10 +981,12 @@ extern "C" int hd_scatter(double* dens, int M, int Q, int Z, int G, const double
                            pe_geo, fdiff, ecap, levels_ptr, n_levels, tol, maxiter, ybuf, grad, iters_out);
     int rc = check_launch("k_ct_gradients");
     if (rc) return rc;
+    }
+evaluate:
     dim3 g1((unsigned)cdiv((int64_t)G * G, 256), Z);
     hipLaunchKernelGGL(k_ct_eval, g1, dim3(256), 0, st, dens, N, Z, G, pts, simplices, simplex_g, q_simplex, q_bary,
                        q_nearest, grad, Dm, bad);
-    rc = check_launch("k_ct_eval");
+    int rc = check_launch("k_ct_eval");
     if (rc) return rc;
     // Y (G x Z*G) = P (G x G) * D (G x Z*G)
     rc = hd_gemm_f64(P, Dm, Y, G, Z * G, G, stream);

@@ -152,8 +152,15 @@ def build_geometry(mtot, mrat, refine=4, kd_workers=-1):
     nlev = int(level.max()) + 1
     # the sweep runs eight lanes per vertex of a level; entries past a vertex's eighth go to an
     # overflow area
-    max_level_width = int(np.bincount(level, minlength=nlev).max())
+    widths = np.bincount(level, minlength=nlev)
+    max_level_width = int(widths.max())
     n_ovf = int(np.maximum(np.diff(indptr) - 8, 0).sum())
+    # pipelined sweeps: sweep s + 1 may run sweep_period levels behind sweep s (one more than the
+    # largest level difference across an edge); a step then holds the levels l, l + P, ...
+    src = np.repeat(np.arange(N), np.diff(indptr))
+    dl = level[indices] - level[src]
+    sweep_period = int(dl.max()) + 1 if dl.size else 1
+    max_step_width = int(max(widths[r::sweep_period].sum() for r in range(sweep_period)))
     lorder = np.argsort(level, kind='stable')
     lptr = np.concatenate([[0], np.cumsum(np.bincount(level, minlength=nlev))])
     # bilinear back-interpolation (RegularGridInterpolator 'linear') at the nodes
@@ -170,7 +177,7 @@ def build_geometry(mtot, mrat, refine=4, kd_workers=-1):
         simplices=tri.simplices.astype(np.int32), simplex_g=_simplex_g(tri, T), transform=T, mgrid=mgrid,
         back_idx=back_idx.astype(np.int32), back_w=back_w)
     return dict(host=host, M=M, Q=Q, G=G, N=N, n_levels=nlev, max_level_width=max_level_width, n_ovf=n_ovf,
-                mgrid=mgrid)
+                sweep_period=sweep_period, max_step_width=max_step_width, mgrid=mgrid)
 
 
 class ScatterPlan:
@@ -184,6 +191,8 @@ class ScatterPlan:
         self.n_levels = g['n_levels']
         self.max_level_width = g['max_level_width']
         self.n_ovf = g['n_ovf']
+        self.sweep_period = g['sweep_period']
+        self.max_step_width = g['max_step_width']
         self.host = g['host']
         self.mgrid = g['mgrid']
         self._dev = None
@@ -298,6 +307,11 @@ def get_plan(mtot, mrat):
     return plan
 
 
+#: run the Gauss-Seidel sweeps pipelined (sweep s + 1 behind sweep s by the plan's period) when they
+#: fit the device; False runs them one after another (the same results bit for bit)
+PIPELINED_SWEEPS = True
+
+
 def add_scatter_to_masses_device(mtot, mrat, dens, scatter, tol=1e-6, maxiter=400, return_grads=False):
     """In place on the (M, Q, Z) device density; returns the per-slice Gauss-Seidel sweep counts (0 = no
     convergence within maxiter, as the reference's estimator reports).  ``return_grads``: also the
@@ -308,11 +322,12 @@ def add_scatter_to_masses_device(mtot, mrat, dens, scatter, tol=1e-6, maxiter=40
     d['P'] = plan.P_for(scatter)
     M, Q, Z = dens.shape
     assert (M, Q) == (plan.M, plan.Q)
-    ws = D.workspace(_lib.size("hd_scatter_workspace_size", M, Q, Z, plan.G, plan.n_ovf))
+    period = plan.sweep_period if PIPELINED_SWEEPS else 0
+    ws = D.workspace(_lib.size("hd_scatter_workspace_size", M, Q, Z, plan.G, plan.n_ovf, plan.n_levels, period))
     iters = D.empty(Z + 1, dtype=torch.int32)
     from .engine import _stage
     with _stage("scatter"):
-        _scatter_call(dens, M, Q, Z, plan, d, tol, maxiter, ws, iters)
+        _scatter_call(dens, M, Q, Z, plan, d, tol, maxiter, ws, iters, period)
     if not return_grads:
         return iters
     # the gradients are the workspace's first region, at its first 256-byte boundary
@@ -321,9 +336,10 @@ def add_scatter_to_masses_device(mtot, mrat, dens, scatter, tol=1e-6, maxiter=40
     return iters, grads
 
 
-def _scatter_call(dens, M, Q, Z, plan, d, tol, maxiter, ws, iters):
+def _scatter_call(dens, M, Q, Z, plan, d, tol, maxiter, ws, iters, period):
     _lib.call("hd_scatter", D.ptr(dens), M, Q, Z, plan.G, D.ptr(d['pts']), D.ptr(d['nbr_ptr']), D.ptr(d['nbr_idx']),
               D.ptr(d['levels_ptr']), D.ptr(d['levels_idx']), plan.n_levels, plan.max_level_width, plan.n_ovf,
+              int(period), plan.max_step_width,
               D.ptr(d['simplices']),
               D.ptr(d['simplex_g']), D.ptr(d['q_simplex']), D.ptr(d['q_bary']), D.ptr(d['q_nearest']),
               D.ptr(d['P']), D.ptr(d['back_idx']), D.ptr(d['back_w']), float(tol), int(maxiter), D.ptr(ws),

@@ -89,6 +89,11 @@ int hd_zero_stalled(double *dens, const double *zprime, int64_t n, void *stream)
  *   levels_ptr (n_levels+1) / levels_idx (N): Gauss-Seidel level schedule;
  *   max_level_width: the most vertices of one level (at most 128);
  *   n_ovf: adjacency entries beyond the eighth of each vertex, summed;
+ *   sweep_period: one more than the largest level difference across an edge
+ *     (0: the Gauss-Seidel sweeps run one after another); with it sweep s + 1
+ *     starts sweep_period levels behind sweep s (pipelined sweeps, the same
+ *     arithmetic), and max_step_width is the most vertices such a step holds
+ *     (the levels l, l + P, l + 2P, ... together);
  *   simplices (S,3), simplex_g (S,3) cross-boundary derivative directions;
  *   q_simplex (G*G) (-1 outside the hull), q_bary (G*G,3), q_nearest (G*G);
  *   P (G,G) = (W W)^T for the Gaussian-CDF weight matrix W (see hd_gemm_f64);
@@ -100,10 +105,10 @@ int hd_zero_stalled(double *dens, const double *zprime, int64_t n, void *stream)
  * bit 21 when a level has more than max_level_width vertices (nothing
  * computed then).
  * ---------------------------------------------------------------------- */
-size_t hd_scatter_workspace_size(int M, int Q, int Z, int G, int n_ovf);
+size_t hd_scatter_workspace_size(int M, int Q, int Z, int G, int n_ovf, int n_levels, int sweep_period);
 int hd_scatter(double *dens, int M, int Q, int Z, int G, const double *pts, const int32_t *nbr_ptr,
                const int32_t *nbr_idx, const int32_t *levels_ptr, const int32_t *levels_idx, int n_levels,
-               int max_level_width, int n_ovf,
+               int max_level_width, int n_ovf, int sweep_period, int max_step_width,
                const int32_t *simplices, const double *simplex_g, const int32_t *q_simplex, const double *q_bary,
                const int32_t *q_nearest, const double *P, const int32_t *back_idx, const double *back_w, double tol,
                int maxiter, void *workspace, size_t ws_bytes, int32_t *iters_out, void *stream);

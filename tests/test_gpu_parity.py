@@ -165,11 +165,14 @@ def test_scatter_and_density_full_config1(gpu, oracle_lib):
     _close(dens, do, 1e-8, atol=1e-13 * scale, what="scattered density")
 
 
-def test_ct_gradients_bit_exact_vs_scipy(gpu):
+@pytest.mark.parametrize("pipelined", [True, False])
+def test_ct_gradients_bit_exact_vs_scipy(gpu, pipelined):
     """The Gauss-Seidel gradient estimate of every slice equals scipy's CloughTocher2DInterpolator
     gradients (tol 1e-6, maxiter 400) bit for bit, at the full 91 x 81 mass grid: five slices of the
     static density, one with a zeroed quadrant (zero numerators, signed zeros), and the density scaled
-    to 1e-295 and 1e280 (the divisions' exact fallback outside the fast step's range)."""
+    to 1e-295 and 1e280 (the divisions' exact fallback outside the fast step's range).  Both with the
+    sweeps pipelined (sweep s + 1 a period of levels behind sweep s) and one after another; and with
+    maxiter = 3, where most slices stop unconverged (scipy's gradients after three sweeps, count 0)."""
     import scipy.interpolate
     import torch
     from holodeck_amd import device as D, scatter
@@ -183,21 +186,32 @@ def test_ct_gradients_bit_exact_vs_scipy(gpu):
     zq[: zq.shape[0] // 2, : zq.shape[1] // 2] = 0.0
     sl += [zq, base[:, :, Zall // 2] * 1e-295, base[:, :, Zall // 2] * 1e280]
     dens = np.stack(sl, axis=-1)
-    print(f"[ct gradients {dens.shape}]", flush=True)
-    d = D.to_dev(dens)
-    iters, grads = scatter.add_scatter_to_masses_device(sam.mtot, sam.mrat, d, 0.28, return_grads=True)
-    torch.cuda.synchronize()
-    g = D.to_host(grads)
-    it = D.to_host(iters)
     m1 = sam.mtot[:, None] / (1.0 + sam.mrat[None, :])
     m2 = sam.mtot[:, None] - m1
     pts = (np.log10(m1.ravel()), np.log10(m2.ravel()))
-    for z in range(dens.shape[-1]):
-        want = np.asarray(scipy.interpolate.CloughTocher2DInterpolator(pts, dens[:, :, z].ravel()).grad)
-        want = want.reshape(-1, 2)
-        diff = np.flatnonzero(g[z].view(np.int64) != want.view(np.int64))
-        assert diff.size == 0, (z, it[z], diff.size, diff[:4], g[z].ravel()[diff[:4]], want.ravel()[diff[:4]])
-        assert it[z] > 0
+    keep = scatter.PIPELINED_SWEEPS
+    scatter.PIPELINED_SWEEPS = pipelined
+    try:
+        for maxiter in (400, 3):
+            print(f"[ct gradients {dens.shape} pipelined={pipelined} maxiter={maxiter}]", flush=True)
+            d = D.to_dev(dens)
+            iters, grads = scatter.add_scatter_to_masses_device(sam.mtot, sam.mrat, d, 0.28, maxiter=maxiter,
+                                                                return_grads=True)
+            torch.cuda.synchronize()
+            g = D.to_host(grads)
+            it = D.to_host(iters)
+            for z in range(dens.shape[-1]):
+                want = np.asarray(scipy.interpolate.CloughTocher2DInterpolator(pts, dens[:, :, z].ravel(),
+                                                                               maxiter=maxiter).grad)
+                want = want.reshape(-1, 2)
+                diff = np.flatnonzero(g[z].view(np.int64) != want.view(np.int64))
+                assert diff.size == 0, (maxiter, z, it[z], diff.size, diff[:4], g[z].ravel()[diff[:4]],
+                                        want.ravel()[diff[:4]])
+                assert (it[z] > 0) if maxiter == 400 else (0 <= it[z] <= 3), (maxiter, z, it[z])
+            if maxiter == 3:   # the stop at maxiter is taken (slices that need more sweeps report 0)
+                assert np.any(it[:-1] == 0), it
+    finally:
+        scatter.PIPELINED_SWEEPS = keep
 
 
 @pytest.mark.parametrize("shape", [(17, 15), (61, 51), (91, 81)])
