@@ -84,8 +84,9 @@ def _transforms(pts, simplices):
         T[:, 1, 1] = a / det
     T[:, 2, :] = P[:, 2, :]
     n1 = np.maximum(np.abs(a) + np.abs(c), np.abs(b) + np.abs(d))   # 1-norm of the matrix
-    ni = np.maximum(np.abs(d) + np.abs(c), np.abs(b) + np.abs(a)) / np.abs(det)
-    bad = ~(1.0 / (n1 * ni) >= np.finfo(float).eps)
+    with np.errstate(divide='ignore', invalid='ignore'):        # degenerate triangles: det = 0
+        ni = np.maximum(np.abs(d) + np.abs(c), np.abs(b) + np.abs(a)) / np.abs(det)
+        bad = ~(1.0 / (n1 * ni) >= np.finfo(float).eps)
     T[bad] = np.nan
     return T
 

@@ -99,7 +99,14 @@ def trapz_grid_weight(index, size, grid):
 def _walk(ndens, mtot_log10, mrat, redz, dcom, gwfobs, sepa_evo, eccen_evo, nharms, visit):
     """The loop nest shared by both reference kernels (cyutils.pyx:466-585, 713-838); calls
     visit(ii, kk, aa, bb, nh, gwfr, sa_fourth, ecc, mt, m1, m2, mchirp, zterm, dc_cm, dc_term, jw)
-    with jw the mass-ratio weights (Q, 2) for every contributing (mass, redshift, f, harmonic)."""
+    with jw the mass-ratio weights (Q, 2) for every contributing (mass, redshift, f, harmonic).
+
+    (mass, redshift) cells whose density is zero at every mass ratio are not walked: every term
+    they would add is proportional to ndens[ii, :, kk], so they add exact zeros (and Poisson means
+    of zero).  The evolution index a skipped cell would have recorded at its first (f, harmonic)
+    is only the starting point of the next, lighter mass's forward walk; the index from a
+    heavier mass is a lower bound of it (rest-frame frequencies of a lighter binary reach each
+    separation later), and the forward walk from any lower bound stops at the same index."""
     nfreqs, n_mtot, n_mrat, n_redz, n_eccs = len(gwfobs), len(mtot_log10), len(mrat), len(redz), len(sepa_evo)
     kep_sa_term = MY_NWTG / pow(2.0 * np.pi, 2)
     freq_pref = (1.0 / (2.0 * np.pi)) * np.sqrt(MY_NWTG)
@@ -117,6 +124,8 @@ def _walk(ndens, mtot_log10, mrat, redz, dcom, gwfobs, sepa_evo, eccen_evo, nhar
         ecc_idx_beg = 0
         for ii_mm in range(n_mtot):
             ii = n_mtot - 1 - ii_mm
+            if not np.any(ndens[ii, :, kk]):
+                continue
             mt = pow(10.0, mtot_log10[ii])
             mt_sqrt = np.sqrt(mt)
             kep_sa_mass_term = kep_sa_term * mt
