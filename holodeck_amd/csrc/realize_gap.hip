@@ -65,7 +65,7 @@ namespace hd {
 
 constexpr int GZ_TILE = 512;          // bins per tile (prep workgroup size)
 constexpr int GZ_REC = GZ_TILE + 1;   // record slots per tile (+1 sparse pseudo-record)
-constexpr int GZ_TILES_PER_CHUNK = 16;
+constexpr int GZ_TILES_PER_CHUNK = 8;    // items of 8 tiles: short enough to pack the GPU to its end
 constexpr int GZ_MAXBS = 512;         // draw workgroup: one realization per thread
 constexpr int GZ_BATCH_REC = 128;     // at most this many records per LDS batch (their headers are staged)
 constexpr int GZ_XCDS = 8;            // MI355X: 8 XCDs, workgroups dispatched round-robin
@@ -120,6 +120,8 @@ struct GapArgs {
     uint64_t* p_key;        // (F, nchunks, L, R): realization fastest
     int32_t* p_bin;         // (F, nchunks, L, R)
     double* counts_out;     // (B, F, R) every drawn count (test hook only; null in production)
+    int32_t* plan;          // (nchunks * F) draw workgroup -> (chunk, f) item, f * nchunks + chunk
+    int32_t* plan_b;        // (nchunks * F) the item's cost bucket
 };
 
 __device__ __forceinline__ uint64_t gz_hkey(double h) {
@@ -323,6 +325,57 @@ __device__ __forceinline__ void gz_record(const GapArgs& A, uint64_t word, int f
 __global__ void k_gz_zero(double* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// plan: the draw workgroups' dispatch order
+// ---------------------------------------------------------------------------
+// A draw workgroup's time follows its (chunk, f) item's work: about 0.7 us per
+// dense record, 1.5 ns per table entry and 1.6 us per expected sparse point
+// (a least-squares fit to workgroup times of the 91x81x101, F=40, R=500
+// workload, R^2 0.94), and the items differ by 10x.  Dispatched in (chunk, f)
+// order, long items that start late leave most of the GPU idle at the end, so
+// the items go out by decreasing cost instead: a counting sort into buckets of
+// 2^(1/4) in cost (the longest-processing-time-first rule, to within that
+// ratio).  The order within a bucket follows atomics and may vary between
+// calls; results do not depend on it (each item writes its own partials from
+// streams keyed by tile, f and realization).
+constexpr int GZ_PLAN_BUCKETS = 128;
+constexpr int GZ_PLAN_THREADS = 1024;
+__global__ void __launch_bounds__(GZ_PLAN_THREADS) k_gz_plan(GapArgs A) {
+    __shared__ int s_off[GZ_PLAN_BUCKETS];
+    const int tid = threadIdx.x;
+    const int n = A.nchunks * A.F;
+    for (int i = tid; i < GZ_PLAN_BUCKETS; i += GZ_PLAN_THREADS) s_off[i] = 0;
+    __syncthreads();
+    for (int it = tid; it < n; it += GZ_PLAN_THREADS) {
+        const int f = it / A.nchunks, chunk = it - f * A.nchunks;
+        const int t0 = chunk * GZ_TILES_PER_CHUNK, t1 = min(t0 + GZ_TILES_PER_CHUNK, A.ntiles);
+        double cost = 1.0;
+        for (int tile = t0; tile < t1; tile++) {
+            const int64_t ft = (int64_t)f * A.ntiles + tile;
+            const int nd = A.cnt[ft * 3], ns = A.cnt[ft * 3 + 1];
+            const int nrec = nd + (ns > 0 ? 1 : 0);
+            cost += 1000.0 * nd;
+            if (nrec > 0) cost += 2.0 * A.rec_end[ft * GZ_REC + nrec - 1];
+            if (ns > 0) cost += 2300.0 * A.s_cum[ft * GZ_TILE + ns - 1];
+        }
+        const int b = (int)fmin(fmax((double)(GZ_PLAN_BUCKETS - 1) - floor(4.0 * log2(cost)), 0.0),
+                                (double)(GZ_PLAN_BUCKETS - 1));
+        A.plan_b[it] = b;
+        atomicAdd(&s_off[b], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {   // exclusive scan of the bucket sizes (heaviest bucket first)
+        int sum = 0;
+        for (int i = 0; i < GZ_PLAN_BUCKETS; i++) {
+            const int c = s_off[i];
+            s_off[i] = sum;
+            sum += c;
+        }
+    }
+    __syncthreads();
+    for (int it = tid; it < n; it += GZ_PLAN_THREADS) A.plan[atomicAdd(&s_off[A.plan_b[it]], 1)] = it;
 }
 
 // ---------------------------------------------------------------------------
@@ -611,13 +664,18 @@ __device__ __forceinline__ void gz_build_seq(uint32_t* hi, double lam, int kb, i
 // step costs one add, one compare and one select, on 4-byte words.)  Two searches
 // run interleaved (gz_draw2) so their LDS round trips overlap.
 struct GzSearch {
-    const uint32_t* hi;
-    const uint32_t* p;   // invariant: the first j with hi[j] >= x is in [p - hi, p - hi + len]
+    const gz_lds_u32* hi;
+    const gz_lds_u32* p;   // invariant: the first j with hi[j] >= x is in [p - hi, p - hi + len]
     int n, len;
     uint32_t x;
 };
 
-__device__ __forceinline__ void gz_search_start(GzSearch& s, const uint32_t* hi, int n, uint32_t x) {
+// Entry index of an LDS word address (32-bit address arithmetic).
+__device__ __forceinline__ int gz_idx(const gz_lds_u32* p, const gz_lds_u32* base) {
+    return (int)(((uint32_t)(size_t)p - (uint32_t)(size_t)base) >> 2);
+}
+
+__device__ __forceinline__ void gz_search_start(GzSearch& s, const gz_lds_u32* hi, int n, uint32_t x) {
     s.hi = hi;
     s.p = hi;
     s.n = n;
@@ -627,7 +685,7 @@ __device__ __forceinline__ void gz_search_start(GzSearch& s, const uint32_t* hi,
 
 __device__ __forceinline__ void gz_search_step(GzSearch& s) {
     const int half = s.len >> 1;
-    const uint32_t* q = s.p + half;
+    const gz_lds_u32* q = s.p + half;
     s.p = (*q < s.x) ? q : s.p;
     s.len -= half;
 }
@@ -637,12 +695,12 @@ template <class Words>
 __device__ __forceinline__ int gz_search_finish(const GzSearch& s, Words& rs, uint32_t& w, bool& has_w) {
     const uint32_t h0 = s.p[0], h1 = s.p[1];
     const bool up = h0 < s.x;
-    int j = (int)(s.p - s.hi) + (up ? 1 : 0);
+    int j = gz_idx(s.p, s.hi) + (up ? 1 : 0);
     has_w = false;
     if (j < s.n && (up ? h1 : h0) == s.x) {
         w = rs.next();
         has_w = true;
-        const uint32_t* lo = s.hi + s.n;
+        const gz_lds_u32* lo = s.hi + s.n;
         while (j < s.n && s.hi[j] == s.x && lo[j] <= w) j++;
     }
     return j;
@@ -692,7 +750,7 @@ __device__ __noinline__ int gz_tail(const gz_lds_u32* hi, int n, int j, uint32_t
 // A table as the draws see it: its words, length, first counts, and the staged
 // header (lam is read from there only by a tail walk).
 struct GzTab {
-    const uint32_t* hi;
+    const gz_lds_u32* hi;
     int n, kb, kt;
     const BRec* rec;
 };
@@ -701,7 +759,7 @@ template <class Words>
 __device__ __forceinline__ int gz_count(const GzTab& t, int j, uint32_t x, uint32_t w, bool has_w, Words& rs) {
     if ((j == 0 && t.kt > t.kb) || j == t.n) {
         if (!has_w) w = rs.next();
-        return gz_tail((const gz_lds_u32*)t.hi, t.n, j, x, w, t.rec->lam, t.kb, t.kt);
+        return gz_tail(t.hi, t.n, j, x, w, t.rec->lam, t.kb, t.kt);
     }
     return t.kt + j;
 }
@@ -716,8 +774,9 @@ __device__ __forceinline__ int gz_draw1(const GzTab& t, Words& rs) {
     {   // the common case, as in gz_draw2
         const uint32_t h0 = s.p[0], h1 = s.p[1];
         const bool up = h0 < x;
-        const int j = (int)(s.p - s.hi) + (up ? 1 : 0);
-        const bool slow = (j < t.n && (up ? h1 : h0) == x) || (j == 0 && t.kt > t.kb) || j == t.n;
+        const int j = gz_idx(s.p, s.hi) + (up ? 1 : 0);
+        // bitwise, so that the test is straight-line code on masks
+        const bool slow = ((j < t.n) & ((up ? h1 : h0) == x)) | ((j == 0) & (t.kt > t.kb)) | (j == t.n);
         if (__ballot(slow) == 0ull) return t.kt + j;
     }
     uint32_t w = 0;
@@ -740,8 +799,8 @@ __device__ __forceinline__ void gz_draw2(const GzTab& ta, const GzTab& tb, Words
     while (a.len > 1 && b.len > 1) {
         // both loads issued before either compare, so their LDS latencies overlap
         const int ha = a.len >> 1, hb = b.len >> 1;
-        const uint32_t* qa = a.p + ha;
-        const uint32_t* qb = b.p + hb;
+        const gz_lds_u32* qa = a.p + ha;
+        const gz_lds_u32* qb = b.p + hb;
         const uint32_t va = *qa;
         const uint32_t vb = *qb;
         a.p = (va < a.x) ? qa : a.p;
@@ -756,10 +815,11 @@ __device__ __forceinline__ void gz_draw2(const GzTab& ta, const GzTab& tb, Words
         // wave-wide test instead of per-lane branches); otherwise the exact path below
         const uint32_t a0 = a.p[0], a1 = a.p[1], b0 = b.p[0], b1 = b.p[1];
         const bool ua = a0 < xa, ub = b0 < xb;
-        const int ja = (int)(a.p - a.hi) + (ua ? 1 : 0), jb = (int)(b.p - b.hi) + (ub ? 1 : 0);
-        const bool sa = (ja < ta.n && (ua ? a1 : a0) == xa) || (ja == 0 && ta.kt > ta.kb) || ja == ta.n;
-        const bool sb = (jb < tb.n && (ub ? b1 : b0) == xb) || (jb == 0 && tb.kt > tb.kb) || jb == tb.n;
-        if (__ballot((ta.n > 0 && sa) || (tb.n > 0 && sb)) == 0ull) {
+        const int ja = gz_idx(a.p, a.hi) + (ua ? 1 : 0), jb = gz_idx(b.p, b.hi) + (ub ? 1 : 0);
+        // bitwise, so that the tests are straight-line code on masks
+        const bool sa = ((ja < ta.n) & ((ua ? a1 : a0) == xa)) | ((ja == 0) & (ta.kt > ta.kb)) | (ja == ta.n);
+        const bool sb = ((jb < tb.n) & ((ub ? b1 : b0) == xb)) | ((jb == 0) & (tb.kt > tb.kb)) | (jb == tb.n);
+        if (__ballot(((ta.n > 0) & sa) | ((tb.n > 0) & sb)) == 0ull) {
             ca = (ta.n > 0) ? ta.kt + ja : 0;
             cb = (tb.n > 0) ? tb.kt + jb : 0;
             return;
@@ -778,7 +838,7 @@ __device__ __forceinline__ void gz_draw2(const GzTab& ta, const GzTab& tb, Words
 __device__ __forceinline__ GzTab gz_tab(const uint32_t* tab, const BRec* b, bool present) {
     GzTab t;
     t.n = present ? __builtin_amdgcn_readfirstlane(b->n) : 0;
-    t.hi = tab + 2 * __builtin_amdgcn_readfirstlane(b->off);
+    t.hi = (const gz_lds_u32*)(tab + 2 * __builtin_amdgcn_readfirstlane(b->off));
     t.kb = __builtin_amdgcn_readfirstlane(b->kb);
     t.kt = __builtin_amdgcn_readfirstlane(b->kt);
     t.rec = b;
@@ -826,8 +886,9 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
     uint32_t* tab = (uint32_t*)gz_lds;   // A.cap 8-byte slots: per record n high words, then n low words
     double* scum = gz_lds + A.cap;     // GZ_TILE doubles
     BRec* brec = (BRec*)(scum + GZ_TILE);   // GZ_BATCH_REC staged record headers
-    const int f = blockIdx.y;
-    const int chunk = blockIdx.x;
+    const int item = A.plan[blockIdx.x];   // heaviest items first
+    const int f = item / A.nchunks;
+    const int chunk = item - f * A.nchunks;
     const int tid = threadIdx.x, lane = tid & 63;
     const int bs = blockDim.x;
     const int F = A.F, R = A.R, L = A.L;
@@ -1036,7 +1097,7 @@ __global__ void __launch_bounds__(256) k_table_draw_hook(const uint32_t* tab, in
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     GzWords ws{words + 2 * i, 0};
-    count[i] = gz_draw1(GzTab{t, n, kb, kt, &hdr}, ws);
+    count[i] = gz_draw1(GzTab{(const gz_lds_u32*)t, n, kb, kt, &hdr}, ws);
     used[i] = ws.i;
 }
 
@@ -1059,7 +1120,8 @@ __global__ void __launch_bounds__(256) k_table_draw_pair_hook(const uint32_t* ta
     if (i >= m) return;
     GzWords ws{words + 4 * i, 0};
     int ca, cb;
-    gz_draw2(GzTab{t, n, kb, kt, &hdr[0]}, GzTab{t2, n2, kb2, kt2, &hdr[1]}, ws, ca, cb);
+    gz_draw2(GzTab{(const gz_lds_u32*)t, n, kb, kt, &hdr[0]}, GzTab{(const gz_lds_u32*)t2, n2, kb2, kt2, &hdr[1]}, ws, ca,
+             cb);
     count[2 * i] = ca;
     count[2 * i + 1] = cb;
     used[i] = ws.i;
@@ -1109,6 +1171,7 @@ static size_t gz_ws(int64_t B, int F, int R, int L, int npar) {
     s += gz_align(pr * 8) + gz_align(pr * 8 * (size_t)npar) + gz_align(pr * 8 * (size_t)L) +
          gz_align(pr * 4 * (size_t)L);
     s += gz_align(realize_group_ws(F, R, L, npar));          // first merge stage
+    s += 2 * gz_align((size_t)nch * F * 4);                  // dispatch plan, cost buckets
     return s + 512;
 }
 
@@ -1116,7 +1179,7 @@ template <int MODE, int LM, int NP, bool REC>
 static int gz_launch_k(const GapArgs& A, dim3 grid, int bs, size_t lds, hipStream_t st) {
     {
         KTimer timer("k_gz_draw", st);
-        hipLaunchKernelGGL((k_gz_draw<MODE, LM, NP, REC>), grid, dim3(bs), lds, st, A);
+        hipLaunchKernelGGL((k_gz_draw<MODE, LM, NP, REC>), dim3(grid.x * grid.y), dim3(bs), lds, st, A);
     }
     int rc = check_launch("k_gz_draw");
     if (rc) return rc;
@@ -1203,7 +1266,9 @@ static int realize_impl(const double* number, const double* h2fdf, const int64_t
     A.p_ptot = (double*)p;        p += gz_align(pr * 8 * (size_t)npar);
     A.p_key = (uint64_t*)p;       p += gz_align(pr * 8 * (size_t)L);
     A.p_bin = (int32_t*)p;        p += gz_align(pr * 4 * (size_t)L);
-    void* group_ws = p;
+    void* group_ws = p;           p += gz_align(realize_group_ws(F, R, L, npar));
+    A.plan = (int32_t*)p;         p += gz_align((size_t)A.nchunks * F * 4);
+    A.plan_b = (int32_t*)p;
     A.rank = (mode == HD_SEL_SORTED) ? rank : nullptr;
     if (mode == HD_SEL_SORTED) {
         int rc = rank_from_order_launch(order, B, rank, st);
@@ -1216,8 +1281,12 @@ static int realize_impl(const double* number, const double* h2fdf, const int64_t
     }
     const int64_t prep_blocks = cdiv(A.ntiles, GZ_XCDS) * GZ_XCDS * (int64_t)F;
     if (prep_blocks > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: too many tiles x frequencies");
+    if ((int64_t)A.nchunks * F > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: too many chunks x frequencies");
     hipLaunchKernelGGL(k_gz_prep, dim3((unsigned)prep_blocks), dim3(GZ_TILE), 0, st, A);
     int rc = check_launch("k_gz_prep");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gz_plan, dim3(1), dim3(GZ_PLAN_THREADS), 0, st, A);
+    rc = check_launch("k_gz_plan");
     if (rc) return rc;
     const dim3 grid(A.nchunks, F);
     const size_t lds = GZ_DRAW_LDS;
