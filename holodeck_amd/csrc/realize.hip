@@ -171,18 +171,27 @@ __device__ __forceinline__ void merge_chunks(const RealizeArgs& A, int f, int r,
         total += A.p_total[pr];
         for (int k = 0; k < A.npar; k++) ptot[k] += A.p_ptot[(int64_t)k * F * NC * R + pr];
         if (use_list) {
-            for (int j = 0; j < L; j++) {
-                const int64_t o = (((int64_t)f * NC + c) * L + j) * R + r;
-                uint64_t k = A.p_key[o];
-                int32_t b = A.p_bin[o];
-                if (k == KEY_NONE) break;
+            // the chunk's whole list first: every load issued before any is used (indices clamped
+            // to slot L-1, no branch), so a chunk costs one load latency instead of one per slot
+            const int64_t o0 = ((int64_t)f * NC + c) * L * (int64_t)R + r;
+            uint64_t ck[LMAX];
+            int32_t cb[LMAX];
+#pragma unroll
+            for (int j = 0; j < LMAX; j++) {
+                const int64_t o = o0 + (int64_t)min(j, L - 1) * R;
+                ck[j] = A.p_key[o];
+                cb[j] = A.p_bin[o];
+            }
+#pragma unroll
+            for (int j = 0; j < LMAX; j++) {
+                if (j >= L || ck[j] == KEY_NONE) break;
                 uint64_t wk = KEY_NONE;
                 int32_t wb = 0x7fffffff;
 #pragma unroll
                 for (int q = 0; q < LMAX; q++)
                     if (q == L - 1) { wk = lk[q]; wb = lb[q]; }
-                if (!better(k, b, wk, wb)) break;
-                list_insert(lk, lb, k, b);
+                if (!better(ck[j], cb[j], wk, wb)) break;
+                list_insert(lk, lb, ck[j], cb[j]);
             }
         }
     }

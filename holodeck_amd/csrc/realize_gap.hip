@@ -122,6 +122,7 @@ struct GapArgs {
     double* counts_out;     // (B, F, R) every drawn count (test hook only; null in production)
     int32_t* plan;          // (nchunks * F) draw workgroup -> (chunk, f) item, f * nchunks + chunk
     int32_t* plan_b;        // (nchunks * F) the item's cost bucket
+    float* tile_cost;       // (F, ntiles) estimated draw time of the tile (prep)
 };
 
 __device__ __forceinline__ uint64_t gz_hkey(double h) {
@@ -196,6 +197,13 @@ __device__ __forceinline__ double gz_bd0(double k, double lam) {
 __device__ __forceinline__ double gz_pmf_acc(double lam, double k) {
     if (k == 0.0) return exp(-lam);
     return exp(-gz_stirling_err(k) - gz_bd0(k, lam)) * rsqrt(2.0 * M_PI * k);
+}
+
+// Estimated draw time of a tile in units of 1 ns: about 0.7 us per dense record, 1.5 ns per
+// table entry and 1.6 us per expected sparse point (a least-squares fit to the draw
+// workgroups' times at 91x81x101, F=40, R=500, R^2 0.94); the dispatch plan sorts by it.
+__device__ __forceinline__ double gz_tile_cost(int nd, int ntab, double lam_sparse) {
+    return 700.0 * nd + 1.5 * ntab + 1600.0 * lam_sparse;
 }
 
 // ---------------------------------------------------------------------------
@@ -312,6 +320,9 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
         A.cnt[ft * 3] = nd;
         A.cnt[ft * 3 + 1] = ns;
         A.cnt[ft * 3 + 2] = nbig;
+        double lam_sparse = 0.0;
+        for (int w = 0; w < GZ_TILE / 64; w++) lam_sparse += s_wsum[w];
+        A.tile_cost[ft] = (float)gz_tile_cost(nd, ntot, lam_sparse);
     }
 }
 
@@ -330,10 +341,8 @@ __global__ void k_gz_zero(double* p, int64_t n) {
 // ---------------------------------------------------------------------------
 // plan: the draw workgroups' dispatch order
 // ---------------------------------------------------------------------------
-// A draw workgroup's time follows its (chunk, f) item's work: about 0.7 us per
-// dense record, 1.5 ns per table entry and 1.6 us per expected sparse point
-// (a least-squares fit to workgroup times of the 91x81x101, F=40, R=500
-// workload, R^2 0.94), and the items differ by 10x.  Dispatched in (chunk, f)
+// A draw workgroup's time follows its (chunk, f) item's work (the sum of its
+// tiles' gz_tile_cost), and the items differ by 10x.  Dispatched in (chunk, f)
 // order, long items that start late leave most of the GPU idle at the end, so
 // the items go out by decreasing cost instead: a counting sort into buckets of
 // 2^(1/4) in cost (the longest-processing-time-first rule, to within that
@@ -350,16 +359,12 @@ __global__ void __launch_bounds__(GZ_PLAN_THREADS) k_gz_plan(GapArgs A) {
     __syncthreads();
     for (int it = tid; it < n; it += GZ_PLAN_THREADS) {
         const int f = it / A.nchunks, chunk = it - f * A.nchunks;
-        const int t0 = chunk * GZ_TILES_PER_CHUNK, t1 = min(t0 + GZ_TILES_PER_CHUNK, A.ntiles);
+        const int t0 = chunk * GZ_TILES_PER_CHUNK;
+        const float* tc = A.tile_cost + (int64_t)f * A.ntiles;
         double cost = 1.0;
-        for (int tile = t0; tile < t1; tile++) {
-            const int64_t ft = (int64_t)f * A.ntiles + tile;
-            const int nd = A.cnt[ft * 3], ns = A.cnt[ft * 3 + 1];
-            const int nrec = nd + (ns > 0 ? 1 : 0);
-            cost += 1000.0 * nd;
-            if (nrec > 0) cost += 2.0 * A.rec_end[ft * GZ_REC + nrec - 1];
-            if (ns > 0) cost += 2300.0 * A.s_cum[ft * GZ_TILE + ns - 1];
-        }
+#pragma unroll
+        for (int k = 0; k < GZ_TILES_PER_CHUNK; k++)   // contiguous, every load issued first
+            cost += (t0 + k < A.ntiles) ? (double)tc[min(t0 + k, A.ntiles - 1)] : 0.0;
         const int b = (int)fmin(fmax((double)(GZ_PLAN_BUCKETS - 1) - floor(4.0 * log2(cost)), 0.0),
                                 (double)(GZ_PLAN_BUCKETS - 1));
         A.plan_b[it] = b;
@@ -538,56 +543,66 @@ __device__ __forceinline__ void gz_fixed64(double c, uint32_t& h, uint32_t& l) {
     l = (uint32_t)((t - (double)h) * 4294967296.0);   // the fraction is exact, so is its scaling
 }
 
-// Wave-wide inclusive scans of doubles by DPP moves (row shifts by 1, 2, 4, 8
-// within each 16-lane row, then the broadcasts of lane 15 and lane 31 to the
-// rows after them), in registers instead of LDS permutes.
-template <int CTRL>
-__device__ __forceinline__ double gz_dpp(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+// Inclusive scans of doubles over groups of G lanes (G = 16: each DPP row; G = 64:
+// the wave) by DPP moves: row shifts by 1, 2, 4, 8, then for G = 64 the broadcasts
+// of lane 15 and lane 31 to the rows after them; in registers instead of LDS
+// permutes.  A lane whose source lies outside its group receives the operation's
+// identity through the DPP "old" operand (rows outside the row mask keep it too),
+// so no selects are needed.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double gz_dpp_id(double v, double id) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(id), __double2loint(v), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(id), __double2hiint(v), CTRL, ROWS, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 
-template <class Op>
-__device__ __forceinline__ double gz_wave_scan(double v, int lane, Op op) {
-    const int rl = lane & 15;
-    double t;
-    t = gz_dpp<0x111>(v);
-    if (rl >= 1) v = op(t, v);
-    t = gz_dpp<0x112>(v);
-    if (rl >= 2) v = op(t, v);
-    t = gz_dpp<0x114>(v);
-    if (rl >= 4) v = op(t, v);
-    t = gz_dpp<0x118>(v);
-    if (rl >= 8) v = op(t, v);
-    t = gz_dpp<0x142>(v);   // row_bcast:15
-    if ((lane & 31) >= 16) v = op(t, v);
-    t = gz_dpp<0x143>(v);   // row_bcast:31
-    if (lane >= 32) v = op(t, v);
+template <int G, class Op>
+__device__ __forceinline__ double gz_group_scan(double v, double id, Op op) {
+    v = op(gz_dpp_id<0x111>(v, id), v);   // row_shr:1
+    v = op(gz_dpp_id<0x112>(v, id), v);   // row_shr:2
+    v = op(gz_dpp_id<0x114>(v, id), v);   // row_shr:4
+    v = op(gz_dpp_id<0x118>(v, id), v);   // row_shr:8
+    if (G == 64) {
+        v = op(gz_dpp_id<0x142, 0xA>(v, id), v);   // row_bcast:15 into rows 1 and 3
+        v = op(gz_dpp_id<0x143, 0xC>(v, id), v);   // row_bcast:31 into rows 2 and 3
+    }
     return v;
+}
+
+// the value of the previous lane of the group (the identity on the group's first lane)
+template <int G>
+__device__ __forceinline__ double gz_group_prev(double v, double id) {
+    return (G == 64) ? gz_dpp_id<0x138>(v, id) : gz_dpp_id<0x111>(v, id);   // wave_shr:1, row_shr:1
 }
 
 struct GzMul { __device__ double operator()(double a, double b) const { return a * b; } };
 struct GzAdd { __device__ double operator()(double a, double b) const { return a + b; } };
 struct GzMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
 
-// Build the CDF table of one record with one wave, as 64-bit fixed point:
+// Build the CDF table of a record with a group of G lanes (G = 16: four records
+// per wave, one per DPP row; G = 64: one record per wave), as 64-bit fixed point:
 // hi[0, n) the high words, hi[n, 2n) the low words, entry j = floor(2^64 CDF(kt + j))
-// with the CDF summed from kb.  The counts kb .. kt + n - 1 are split into contiguous
-// lane segments; p(kb) = p0 comes from the prep kernel and the pmf follows
-// p(k+1) = p(k) lam / (k+1).  Pass 1: each lane's pmf relative to its segment's
-// first count, their product and their sum; a wave prefix product gives each
-// segment's first pmf, a wave prefix sum of the segment sums the carries.  Pass 2
-// repeats the relative sums and writes carry + first pmf * relative sum for the
-// counts from kt on (lanes wholly below kt write nothing: the lower tail is only
-// summed).  (Relative error of the CDF about (kt + n - kb) ulp, then rounding down
-// to a multiple of 2^-64.)
+// with the CDF summed from kb.  The counts kb .. kt + n - 1 are split into
+// contiguous segments of the group's lanes; p(kb) = p0 comes from the prep kernel
+// and the pmf follows p(k+1) = p(k) lam / (k+1).  Pass 1: each lane's pmf relative
+// to its segment's first count, their product and their sum; a group prefix product
+// gives each segment's first pmf, a group prefix sum of the segment sums the
+// carries.  Pass 2 repeats the relative sums and writes carry + first pmf * relative
+// sum (one rounding, fma) for the counts from kt on (lanes wholly below kt write
+// nothing: the lower tail is only summed).  (Relative error of the CDF about
+// (kt + n - kb) ulp, then rounding down to a multiple of 2^-64.)  Four records per
+// wave share the fixed cost of the scans; long tables take a whole wave, which
+// shortens each lane's serial recurrence (the builds of a batch are its critical
+// path).  Arguments are the group's record (n = 0 and kt = kb: no table, the group
+// idles); gl = lane index within the group; the whole wave calls (the scans read
+// every lane).
 typedef __attribute__((address_space(3))) uint32_t gz_lds_u32;
-__device__ __noinline__ void gz_build(gz_lds_u32* hi, double lam, int kb, int kt, int n, double p0, int lane) {
+template <int G>
+__device__ __noinline__ void gz_build(gz_lds_u32* hi, double lam, int kb, int kt, int n, double p0, int gl) {
     gz_lds_u32* lo = hi + n;
     const int nb = kt + n - kb;          // counts summed
-    const int s = (nb + 63) >> 6;
-    const int j0 = lane * s;
+    const int s = (nb + G - 1) / G;
+    const int j0 = gl * s;
     const int m = min(max(nb - j0, 0), s);
     const double kf = (double)(kb + j0);
     double prod = 1.0, rel = 0.0;        // pass 1
@@ -595,24 +610,19 @@ __device__ __noinline__ void gz_build(gz_lds_u32* hi, double lam, int kb, int kt
         rel += prod;
         prod *= lam * gz_rcp(kf + (double)(j + 1));
     }
-    const double px = gz_wave_scan(prod, lane, GzMul());
-    double pre = __shfl_up(px, 1, 64);   // product over the lanes before this one
-    if (lane == 0) pre = 1.0;
+    const double pre = gz_group_prev<G>(gz_group_scan<G>(prod, 1.0, GzMul()), 1.0);   // over the lanes before
     const double pstart = p0 * pre;      // p(kb + j0)
     const double local = pstart * rel;
-    const double x = gz_wave_scan(local, lane, GzAdd());
-    double carry = __shfl_up(x, 1, 64);  // exclusive prefix: the sum over the lanes before this one
-    if (lane == 0) carry = 0.0;
-    const double last = (m > 0) ? local + carry : -1.0;
+    const double carry = gz_group_prev<G>(gz_group_scan<G>(local, 0.0, GzAdd()), 0.0);   // sum of those before
+    const double last = (m > 0) ? __fma_rn(pstart, rel, carry) : -1.0;
     // The carries are rounded independently of the segment sums, so two
     // neighbouring segments can meet out of order by an ulp; raise each
     // segment to the largest value before it, which makes the table exactly
     // non-decreasing (so the inversion is a monotone map of the uniform).
-    const double mx = gz_wave_scan(last, lane, GzMax());
-    double before = __shfl_up(mx, 1, 64);
-    if (lane == 0) before = 0.0;
-    // pass 2, for segments that reach the table; the same relative sums, so a
-    // segment's last value is exactly `last`
+    const double ninf = -__builtin_inf();
+    const double before = gz_group_prev<G>(gz_group_scan<G>(last, ninf, GzMax()), ninf);
+    // pass 2, for segments that reach the table; the same relative sums and the same fma,
+    // so a segment's last value is exactly `last`
     const int jt = kt - kb - j0;         // segment index of the table's first count
     if (jt < m) {
         double q = 1.0, r = 0.0;
@@ -620,7 +630,7 @@ __device__ __noinline__ void gz_build(gz_lds_u32* hi, double lam, int kb, int kt
             r += q;
             if (j >= jt) {
                 uint32_t h, l;
-                gz_fixed64(fmax(pstart * r + carry, before), h, l);
+                gz_fixed64(fmax(__fma_rn(pstart, r, carry), before), h, l);
                 hi[j0 + j - (kt - kb)] = h;
                 lo[j0 + j - (kt - kb)] = l;
             }
@@ -629,12 +639,21 @@ __device__ __noinline__ void gz_build(gz_lds_u32* hi, double lam, int kb, int kt
     }
 }
 
+// Counts summed (kt + n - kb) up to which a table is built by a 16-lane row, four per wave;
+// longer tables take a whole wave.
+#ifndef HD_GZ_ROW_MAX
+#define HD_GZ_ROW_MAX 256
+#endif
+
 // A short table built by one thread in one pass: the same entries as gz_build
 // (the CDF summed from kb by the pmf recurrence, floor(2^64 CDF) as high and low
 // words), the sum running in order, so the entries are non-decreasing as they
 // are.  For tables whose counts kb .. kt + n - 1 number at most GZ_SEQ_MAX: a
 // thread per table avoids the wave scans, which cost more than such a table.
-constexpr int GZ_SEQ_MAX = 32;
+#ifndef HD_GZ_SEQ_MAX
+#define HD_GZ_SEQ_MAX 32
+#endif
+constexpr int GZ_SEQ_MAX = HD_GZ_SEQ_MAX;
 __device__ __forceinline__ void gz_build_seq(uint32_t* hi, double lam, int kb, int kt, int n, double p0) {
     uint32_t* lo = hi + n;
     double p = p0, c = 0.0;
@@ -775,9 +794,11 @@ __device__ __forceinline__ int gz_draw1(const GzTab& t, Words& rs) {
         const uint32_t h0 = s.p[0], h1 = s.p[1];
         const bool up = h0 < x;
         const int j = gz_idx(s.p, s.hi) + (up ? 1 : 0);
-        // bitwise, so that the test is straight-line code on masks
-        const bool slow = ((j < t.n) & ((up ? h1 : h0) == x)) | ((j == 0) & (t.kt > t.kb)) | (j == t.n);
-        if (__ballot(slow) == 0ull) return t.kt + j;
+        // fast: no tie on the high word and j in [lo, n), lo = 1 when entry 0 carries a summed
+        // lower tail (kt > kb), else 0 -- one unsigned compare for the range
+        const int lo = (t.kt > t.kb) ? 1 : 0;
+        const bool fast = ((uint32_t)(j - lo) < (uint32_t)(t.n - lo)) & ((up ? h1 : h0) != x);
+        if (__builtin_amdgcn_ballot_w64(!fast) == 0ull) return t.kt + j;
     }
     uint32_t w = 0;
     bool hw;
@@ -816,10 +837,12 @@ __device__ __forceinline__ void gz_draw2(const GzTab& ta, const GzTab& tb, Words
         const uint32_t a0 = a.p[0], a1 = a.p[1], b0 = b.p[0], b1 = b.p[1];
         const bool ua = a0 < xa, ub = b0 < xb;
         const int ja = gz_idx(a.p, a.hi) + (ua ? 1 : 0), jb = gz_idx(b.p, b.hi) + (ub ? 1 : 0);
-        // bitwise, so that the tests are straight-line code on masks
-        const bool sa = ((ja < ta.n) & ((ua ? a1 : a0) == xa)) | ((ja == 0) & (ta.kt > ta.kb)) | (ja == ta.n);
-        const bool sb = ((jb < tb.n) & ((ub ? b1 : b0) == xb)) | ((jb == 0) & (tb.kt > tb.kb)) | (jb == tb.n);
-        if (__ballot(((ta.n > 0) & sa) | ((tb.n > 0) & sb)) == 0ull) {
+        // fast: no tie on a high word and j in [lo, n) (lo = 1 when entry 0 carries a summed lower
+        // tail), as in gz_draw1; straight-line code on masks
+        const int loa = (ta.kt > ta.kb) ? 1 : 0, lob = (tb.kt > tb.kb) ? 1 : 0;
+        const bool fa = ((uint32_t)(ja - loa) < (uint32_t)(ta.n - loa)) & ((ua ? a1 : a0) != xa);
+        const bool fb = ((uint32_t)(jb - lob) < (uint32_t)(tb.n - lob)) & ((ub ? b1 : b0) != xb);
+        if (__builtin_amdgcn_ballot_w64(((ta.n > 0) & !fa) | ((tb.n > 0) & !fb)) == 0ull) {
             ca = (ta.n > 0) ? ta.kt + ja : 0;
             cb = (tb.n > 0) ? tb.kt + jb : 0;
             return;
@@ -880,9 +903,13 @@ template <int MODE, int LM, int NP, bool REC>
 // (6 waves per SIMD: with the LDS below three 512-thread workgroups share a CU)
 __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
     extern __shared__ double gz_lds[];
-    __shared__ int s_next;             // next long table of the batch to build
-    __shared__ int s_nlarge;           // long tables of the batch
+    __shared__ int s_next;             // next build item of the batch
+    __shared__ int s_nlarge;           // long tables of the batch (a wave each)
+    __shared__ int s_nrow;             // mid-size tables (a 16-lane row each)
+    __shared__ int s_nshort;           // short tables (a lane each)
     __shared__ int s_large[GZ_BATCH_REC];
+    __shared__ int s_row[GZ_BATCH_REC];
+    __shared__ int s_short[GZ_BATCH_REC];
     uint32_t* tab = (uint32_t*)gz_lds;   // A.cap 8-byte slots: per record n high words, then n low words
     double* scum = gz_lds + A.cap;     // GZ_TILE doubles
     BRec* brec = (BRec*)(scum + GZ_TILE);   // GZ_BATCH_REC staged record headers
@@ -931,33 +958,50 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
                 if (tid == 0) {
                     s_next = 0;
                     s_nlarge = 0;
+                    s_nrow = 0;
+                    s_nshort = 0;
                 }
                 __syncthreads();   // the previous batch's draws are done with tab / scum
-                // one thread per record of the batch stages its header and builds a short table
-                // itself, in one sequential pass; longer tables are listed for the waves
-                {   // record i of the batch on wave i % 8, lane i / 8: every wave takes a share
-                    const int i = (tid & 63) * 8 + (tid >> 6);
-                    if (c + i < c1) {
-                        const DRec rec = A.drec[rreg + c + i];
-                        brec[i] = BRec{rec.h, rec.word, rec.lam, rec.kt, rec.n, rec.off - base, rec.kb, rec.p0};
-                        if (rec.n > 0 && rec.kt + rec.n - rec.kb <= GZ_SEQ_MAX)
-                            gz_build_seq(tab + 2 * (rec.off - base), rec.lam, rec.kb, rec.kt, rec.n, rec.p0);
-                        else if (rec.n > 0)
-                            s_large[atomicAdd(&s_nlarge, 1)] = i;
-                    }
+                // one thread per record of the batch stages its header and lists its table as
+                // short (at most GZ_SEQ_MAX counts summed) or long
+                if (c + tid < c1) {
+                    const DRec rec = A.drec[rreg + c + tid];
+                    brec[tid] = BRec{rec.h, rec.word, rec.lam, rec.kt, rec.n, rec.off - base, rec.kb, rec.p0};
+                    const int nb = rec.kt + rec.n - rec.kb;
+                    if (rec.n > 0 && nb <= GZ_SEQ_MAX) s_short[atomicAdd(&s_nshort, 1)] = tid;
+                    else if (rec.n > 0 && nb <= HD_GZ_ROW_MAX) s_row[atomicAdd(&s_nrow, 1)] = tid;
+                    else if (rec.n > 0) s_large[atomicAdd(&s_nlarge, 1)] = tid;
                 }
                 __syncthreads();
-                // waves take the long tables one at a time (their lengths differ), from the
-                // staged headers
+                // build items handed out to the waves one at a time, the longest serial work
+                // first: long tables (a wave each), then mid-size tables four at a time (a
+                // 16-lane row each), then groups of 64 short tables (a lane each, one
+                // sequential pass), so that every lane a build occupies has work
                 {
-                    const int nl = s_nlarge;
+                    const int nl = s_nlarge, nr = s_nrow, nsh = s_nshort;
+                    const int nrg = (nr + 3) >> 2, nsg = (nsh + 63) >> 6;
                     for (;;) {
-                        int i = 0;
-                        if (lane == 0) i = atomicAdd(&s_next, 1);
-                        i = __shfl(i, 0, 64);
-                        if (i >= nl) break;
-                        const BRec* rec = brec + s_large[i];
-                        gz_build((gz_lds_u32*)(tab + 2 * rec->off), rec->lam, rec->kb, rec->kt, rec->n, rec->p0, lane);
+                        int w = 0;
+                        if (lane == 0) w = atomicAdd(&s_next, 1);
+                        w = __shfl(w, 0, 64);
+                        if (w >= nl + nrg + nsg) break;
+                        if (w < nl) {
+                            const BRec* rec = brec + s_large[w];
+                            gz_build<64>((gz_lds_u32*)(tab + 2 * rec->off), rec->lam, rec->kb, rec->kt, rec->n, rec->p0,
+                                         lane);
+                        } else if (w < nl + nrg) {
+                            const int k = (w - nl) * 4 + (lane >> 4);
+                            const bool on = k < nr;
+                            const BRec* rec = brec + s_row[on ? k : 0];
+                            gz_build<16>((gz_lds_u32*)(tab + 2 * rec->off), rec->lam, on ? rec->kb : 0, on ? rec->kt : 0,
+                                         on ? rec->n : 0, rec->p0, lane & 15);
+                        } else {
+                            const int k = (w - nl - nrg) * 64 + lane;
+                            if (k < nsh) {
+                                const BRec* rec = brec + s_short[k];
+                                gz_build_seq(tab + 2 * rec->off, rec->lam, rec->kb, rec->kt, rec->n, rec->p0);
+                            }
+                        }
                     }
                 }
                 if (c1 == nrec && ns > 0)
@@ -1065,8 +1109,12 @@ __global__ void __launch_bounds__(64) k_cdf_table_hook(double lam, int cap, int*
     if (n > 0) {   // the build the draw kernel uses for this length
         if (kt + n - kb <= GZ_SEQ_MAX) {
             if (lane == 0) gz_build_seq(tab, lam, kb, kt, n, gz_pmf_acc(lam, (double)kb));
+        } else if (kt + n - kb <= HD_GZ_ROW_MAX) {   // row 0 builds, rows 1-3 idle
+            const bool on = lane < 16;
+            gz_build<16>((gz_lds_u32*)tab, lam, on ? kb : 0, on ? kt : 0, on ? n : 0, gz_pmf_acc(lam, (double)kb),
+                         lane & 15);
         } else {
-            gz_build((gz_lds_u32*)tab, lam, kb, kt, n, gz_pmf_acc(lam, (double)kb), lane);
+            gz_build<64>((gz_lds_u32*)tab, lam, kb, kt, n, gz_pmf_acc(lam, (double)kb), lane);
         }
     }
     __syncthreads();
@@ -1152,7 +1200,8 @@ static int gz_cap(int) { return HD_GZ_CAP; }
 // 160 KB CU (with the 64 KB per-workgroup limit of the allocation), and the hook's table as well
 constexpr size_t GZ_DRAW_LDS = (size_t)(HD_GZ_CAP + GZ_TILE) * sizeof(double) + (size_t)GZ_BATCH_REC * sizeof(BRec);
 static_assert(GZ_DRAW_LDS + 16 <= 65536, "draw workgroup LDS above 64 KB");
-static_assert(3 * (GZ_DRAW_LDS + 512) <= 160 * 1024, "three draw workgroups no longer share a CU");
+// (the static part: the build counters and the three 128-entry lists of k_gz_draw, rounded up)
+static_assert(3 * (GZ_DRAW_LDS + 2048) <= 160 * 1024, "three draw workgroups no longer share a CU");
 constexpr int GZ_HOOK_CAP_MAX = 8000;   // hd_poisson_cdf_table: 8-byte slots, 64000 B of LDS
 static_assert(GZ_HOOK_CAP_MAX * sizeof(double) <= 65536, "hook table above 64 KB");
 
@@ -1172,6 +1221,7 @@ static size_t gz_ws(int64_t B, int F, int R, int L, int npar) {
          gz_align(pr * 4 * (size_t)L);
     s += gz_align(realize_group_ws(F, R, L, npar));          // first merge stage
     s += 2 * gz_align((size_t)nch * F * 4);                  // dispatch plan, cost buckets
+    s += gz_align((size_t)F * nt * 4);                       // tile costs
     return s + 512;
 }
 
@@ -1268,7 +1318,8 @@ static int realize_impl(const double* number, const double* h2fdf, const int64_t
     A.p_bin = (int32_t*)p;        p += gz_align(pr * 4 * (size_t)L);
     void* group_ws = p;           p += gz_align(realize_group_ws(F, R, L, npar));
     A.plan = (int32_t*)p;         p += gz_align((size_t)A.nchunks * F * 4);
-    A.plan_b = (int32_t*)p;
+    A.plan_b = (int32_t*)p;       p += gz_align((size_t)A.nchunks * F * 4);
+    A.tile_cost = (float*)p;
     A.rank = (mode == HD_SEL_SORTED) ? rank : nullptr;
     if (mode == HD_SEL_SORTED) {
         int rc = rank_from_order_launch(order, B, rank, st);
