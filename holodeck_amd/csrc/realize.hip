@@ -39,12 +39,27 @@ struct RealizeArgs {
     const double* par;       // (npar,B,F) or null
     int64_t B;
     int F, R, L, mode, flags, npar, nchunks;
-    // partials
+    const int64_t* order;    // (B) bin at each position of the sorted walk (final merge, SORTED)
+    // partials.  The sorted walk keeps only 32-bit keys (its order ranks: unique, so no bin
+    // tie-break, and the bin is order[rank]) in p_key viewed as uint32_t; argmax keeps 64-bit
+    // keys and the bins.
     double* p_total;      // (F, nchunks, R)
     double* p_ptot;       // (npar, F, nchunks, R)
     uint64_t* p_key;      // (F, nchunks, L, R): realization fastest
-    int32_t* p_bin;       // (F, nchunks, L, R)
+    int32_t* p_bin;       // (F, nchunks, L, R), argmax only
 };
+
+constexpr uint32_t KEY32_NONE = 0xffffffffu;
+
+// store / load one slot of a partial list in the mode's layout
+__device__ __forceinline__ void part_store(const RealizeArgs& A, int64_t o, uint64_t k, int32_t b) {
+    if (A.mode == HD_SEL_SORTED) {
+        ((uint32_t*)A.p_key)[o] = (k == KEY_NONE) ? KEY32_NONE : (uint32_t)k;
+    } else {
+        A.p_key[o] = k;
+        A.p_bin[o] = b;
+    }
+}
 
 // ascending image of a double, so that larger h -> smaller key when inverted
 __device__ __forceinline__ uint64_t h_priority(double h) {
@@ -153,8 +168,7 @@ __global__ void __launch_bounds__(512) k_realize_chunk(RealizeArgs A) {
                 for (int j = 0; j < LMAX; j++)
                     if (j < L) {
                         const int64_t o = ((pr / R) * L + j) * R + (pr % R);
-                        A.p_key[o] = lk[j];
-                        A.p_bin[o] = lb[j];
+                        part_store(A, o, lk[j], lb[j]);
                     }
             }
         }
@@ -176,11 +190,21 @@ __device__ __forceinline__ void merge_chunks(const RealizeArgs& A, int f, int r,
             const int64_t o0 = ((int64_t)f * NC + c) * L * (int64_t)R + r;
             uint64_t ck[LMAX];
             int32_t cb[LMAX];
+            if (A.mode == HD_SEL_SORTED) {
+                const uint32_t* pk = (const uint32_t*)A.p_key;
 #pragma unroll
-            for (int j = 0; j < LMAX; j++) {
-                const int64_t o = o0 + (int64_t)min(j, L - 1) * R;
-                ck[j] = A.p_key[o];
-                cb[j] = A.p_bin[o];
+                for (int j = 0; j < LMAX; j++) {
+                    const uint32_t k = pk[o0 + (int64_t)min(j, L - 1) * R];
+                    ck[j] = (k == KEY32_NONE) ? KEY_NONE : (uint64_t)k;
+                    cb[j] = 0;   // ranks are unique: no tie-break
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < LMAX; j++) {
+                    const int64_t o = o0 + (int64_t)min(j, L - 1) * R;
+                    ck[j] = A.p_key[o];
+                    cb[j] = A.p_bin[o];
+                }
             }
 #pragma unroll
             for (int j = 0; j < LMAX; j++) {
@@ -231,8 +255,12 @@ __global__ void __launch_bounds__(256) k_realize_group(RealizeArgs A, int G, dou
         for (int j = 0; j < LMAX; j++)
             if (j < L) {
                 const int64_t o = (((int64_t)f * G + g) * L + j) * R + r;
-                q_key[o] = lk[j];
-                q_bin[o] = lb[j];
+                if (A.mode == HD_SEL_SORTED) {
+                    ((uint32_t*)q_key)[o] = (lk[j] == KEY_NONE) ? KEY32_NONE : (uint32_t)lk[j];
+                } else {
+                    q_key[o] = lk[j];
+                    q_bin[o] = lb[j];
+                }
             }
     }
 }
@@ -270,7 +298,7 @@ __global__ void __launch_bounds__(256) k_realize_merge(RealizeArgs A, double* __
                 hc2ss[o] = 0.0;
                 ssidx[o] = -1;
             } else {
-                int64_t b = lb[j];
+                const int64_t b = (A.mode == HD_SEL_SORTED) ? A.order[lk[j]] : (int64_t)lb[j];
                 double h = A.h2fdf[b * F + f];
                 hc2ss[o] = h;
                 ssidx[o] = b;
@@ -299,8 +327,8 @@ __global__ void k_rank_from_order(const int64_t* __restrict__ order, int64_t B, 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t realize_group_ws(int F, int R, int L, int npar);
-int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int F, int R, int L, int mode, int npar,
-                         int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
+int realize_merge_launch(const double* h2fdf, const double* par, const int64_t* order, int64_t B, int F, int R, int L,
+                         int mode, int npar, int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
                          const int32_t* p_bin, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
                          void* group_ws, hipStream_t st);
 
@@ -357,6 +385,7 @@ static int realize_counts_impl(const double* counts, const double* h2fdf, const 
     A.p_bin = (int32_t*)p;        p += align256(pr * 4 * (size_t)L);
     void* group_ws = p;
     A.rank = (mode == HD_SEL_SORTED) ? rank : nullptr;
+    A.order = order;
 
     if (mode == HD_SEL_SORTED) {
         hipLaunchKernelGGL(k_rank_from_order, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, st, order, B, rank);
@@ -369,8 +398,8 @@ static int realize_counts_impl(const double* counts, const double* h2fdf, const 
     int rc = check_launch("k_realize_chunk");
     if (rc) return rc;
     // the same two-stage merge as the drawn-count kernels (groups of chunks first when there are many)
-    return realize_merge_launch(h2fdf, par, B, F, R, L, mode, npar, A.nchunks, A.p_total, A.p_ptot, A.p_key, A.p_bin,
-                                hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, group_ws, st);
+    return realize_merge_launch(h2fdf, par, order, B, F, R, L, mode, npar, A.nchunks, A.p_total, A.p_ptot, A.p_key,
+                                A.p_bin, hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, group_ws, st);
 }
 
 // Merge of per-chunk partials, shared with the drawn-count kernel (realize_gap.hip).
@@ -379,14 +408,15 @@ size_t realize_group_ws(int F, int R, int L, int npar) {
            4 * 256;
 }
 
-int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int F, int R, int L, int mode, int npar,
-                         int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
+int realize_merge_launch(const double* h2fdf, const double* par, const int64_t* order, int64_t B, int F, int R, int L,
+                         int mode, int npar, int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
                          const int32_t* p_bin, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
                          void* group_ws, hipStream_t st) {
     RealizeArgs A;
     A.counts = nullptr;
     A.h2fdf = h2fdf;
     A.rank = nullptr;
+    A.order = order;
     A.par = par;
     A.B = B;
     A.F = F;

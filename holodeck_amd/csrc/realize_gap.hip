@@ -73,7 +73,10 @@ constexpr uint64_t GZ_KEY_NONE = ~0ull;
 constexpr double GZ_TAILB = 38.816242111356935;   // 56 ln 2: each dropped tail < 2^-56
 constexpr double GZ_CENTRE = 4.0;                 // half-width of a table in standard deviations
 #ifndef HD_GZ_TAU
-#define HD_GZ_TAU 1.0   // lambda below which a bin joins its tile's sparse Poisson process
+// lambda below which a bin joins its tile's sparse Poisson process: a sparse bin costs about
+// lambda point placements per realization, a dense one a table and a draw; the draw launch
+// is fastest for tau in 0.35 .. 0.7 (measured at 91x81x101, F=40, R=500)
+#define HD_GZ_TAU 0.5
 #endif
 
 // dense record (and the sparse pseudo-record at index nd)
@@ -117,8 +120,8 @@ struct GapArgs {
     // partials
     double* p_total;        // (F, nchunks, R)
     double* p_ptot;         // (npar, F, nchunks, R)
-    uint64_t* p_key;        // (F, nchunks, L, R): realization fastest
-    int32_t* p_bin;         // (F, nchunks, L, R)
+    uint64_t* p_key;        // (F, nchunks, L, R): realization fastest; 32-bit ranks for the sorted walk
+    int32_t* p_bin;         // (F, nchunks, L, R), argmax only
     double* counts_out;     // (B, F, R) every drawn count (test hook only; null in production)
     int32_t* plan;          // (nchunks * F) draw workgroup -> (chunk, f) item, f * nchunks + chunk
     int32_t* plan_b;        // (nchunks * F) the item's cost bucket
@@ -424,12 +427,14 @@ struct LoudList<HD_SEL_SORTED, LM> {
     __device__ __forceinline__ uint64_t key_at(int j) const {
         return k[j] == 0xffffffffu ? GZ_KEY_NONE : (uint64_t)k[j];
     }
+    __device__ __forceinline__ uint32_t k32(int j) const { return k[j]; }   // 0xffffffff: none
+    // the list as stored by gz_store (32-bit ranks)
     __device__ __forceinline__ void load(const uint64_t* pk, const int32_t*, int L, int64_t stride) {
         if (gz_exact(LM)) L = LM;
+        const uint32_t* p = (const uint32_t*)pk;
 #pragma unroll
         for (int i = 0; i < LM; i++) {
-            const uint64_t key = (i < L) ? pk[i * stride] : GZ_KEY_NONE;
-            k[i] = (key == GZ_KEY_NONE) ? 0xffffffffu : (uint32_t)key;
+            k[i] = (i < L) ? p[i * stride] : 0xffffffffu;
             if (i == L - 1) wk = k[i];
         }
     }
@@ -891,8 +896,12 @@ __device__ __forceinline__ void gz_store(const GapArgs& A, const Acc<MODE, LM, N
         for (int j = 0; j < LM; j++) {
             if (j >= L) continue;
             const int64_t o = ((pr / R) * L + j) * R + (pr % R);
-            A.p_key[o] = acc.list.key_at(j);
-            A.p_bin[o] = acc.list.bin_at(j, A.order);
+            if constexpr (MODE == HD_SEL_SORTED) {   // ranks only (the merge takes bins from the order)
+                ((uint32_t*)A.p_key)[o] = acc.list.k32(j);
+            } else {
+                A.p_key[o] = acc.list.key_at(j);
+                A.p_bin[o] = acc.list.bin_at(j, A.order);
+            }
         }
     }
 }
@@ -1078,7 +1087,8 @@ __global__ void __launch_bounds__(256) k_gz_big(GapArgs A) {
         for (int k = 0; k < NP; k++) acc.pt[k] = (k < A.npar) ? A.p_ptot[(int64_t)k * F * A.nchunks * R + pr] : 0.0;
         if (MODE != HD_SEL_NONE) {
             const int64_t o = ((pr / R) * L) * R + (pr % R);
-            acc.list.load(A.p_key + o, A.p_bin + o, L, R);
+            acc.list.load(MODE == HD_SEL_SORTED ? (const uint64_t*)((const uint32_t*)A.p_key + o) : A.p_key + o,
+                          A.p_bin + o, L, R);
         }
         for (int tile = t0; tile < t1; tile++) {
             const int64_t ft = (int64_t)f * A.ntiles + tile;
@@ -1175,8 +1185,8 @@ __global__ void __launch_bounds__(256) k_table_draw_pair_hook(const uint32_t* ta
     used[i] = ws.i;
 }
 
-int realize_merge_launch(const double* h2fdf, const double* par, int64_t B, int F, int R, int L, int mode, int npar,
-                         int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
+int realize_merge_launch(const double* h2fdf, const double* par, const int64_t* order, int64_t B, int F, int R, int L,
+                         int mode, int npar, int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
                          const int32_t* p_bin, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
                          void* group_ws, hipStream_t st);
 size_t realize_group_ws(int F, int R, int L, int npar);
@@ -1355,8 +1365,8 @@ static int realize_impl(const double* number, const double* h2fdf, const int64_t
         rc = gz_launch_np<HD_SEL_NONE, 1>(A, npar, grid, bs, lds, st);
     }
     if (rc) return rc;
-    return realize_merge_launch(h2fdf, par, B, F, R, L, mode, npar, A.nchunks, A.p_total, A.p_ptot, A.p_key, A.p_bin,
-                                hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, group_ws, st);
+    return realize_merge_launch(h2fdf, par, order, B, F, R, L, mode, npar, A.nchunks, A.p_total, A.p_ptot, A.p_key,
+                                A.p_bin, hc2ss, ssidx, hc2bg, (npar > 0) ? bgpar : nullptr, group_ws, st);
 }
 
 extern "C" int hd_realize(const double* number, const double* h2fdf, const int64_t* order, int64_t B, int F, int R, int L,
