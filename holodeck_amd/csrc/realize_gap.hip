@@ -212,120 +212,133 @@ __device__ __forceinline__ double gz_tile_cost(int nd, int ntab, double lam_spar
 // ---------------------------------------------------------------------------
 // prep: classify and compact one tile of one frequency
 // ---------------------------------------------------------------------------
+constexpr int GZ_PREP_FG = 8;   // frequencies per prep workgroup
 __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
     __shared__ int s_wd[GZ_TILE / 64], s_ws[GZ_TILE / 64], s_wn[GZ_TILE / 64], s_wb[GZ_TILE / 64];
     __shared__ double s_wsum[GZ_TILE / 64];
-    // XCD-aware order: workgroup w runs on XCD w % 8; the workgroups of one XCD
-    // take one tile for every frequency in turn, so the tile's rows of number
-    // and h2fdf (F doubles each) are fetched once into that XCD's L2.
-    const int w = blockIdx.x;
-    const int k8 = w / GZ_XCDS;
-    const int f = k8 % A.F;
-    const int tile = (k8 / A.F) * GZ_XCDS + (w % GZ_XCDS);
-    if (tile >= A.ntiles) return;
+    // workgroup per (tile, group of GZ_PREP_FG frequencies): each thread reads its bin's
+    // GZ_PREP_FG consecutive values of number and h2fdf (whole cache lines, instead of one
+    // 8-byte word per line and frequency), then the frequencies are classified in turn
+    const int ngroups = (A.F + GZ_PREP_FG - 1) / GZ_PREP_FG;
+    const int tile = blockIdx.x / ngroups;
+    const int f0 = (blockIdx.x - tile * ngroups) * GZ_PREP_FG;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int F = A.F;
     const int64_t b = (int64_t)tile * GZ_TILE + tid;
     const bool skip_zero_h = (A.flags & HD_FLAG_SKIP_ZERO_H) != 0;
-    double lam = 0.0, h = 0.0;
-    bool in = b < A.B;
-    if (in) {
-        lam = A.number[b * F + f];
-        h = A.h2fdf[b * F + f];
-        if (!(lam > 0.0)) in = false;
-        if (A.mode == HD_SEL_SORTED && skip_zero_h && h == 0.0) in = false;
-        // argmax: h <= 0 adds n*h = 0 and can never be selected
-        if (A.mode == HD_SEL_ARGMAX && !(h > 0.0)) in = false;
-    }
-    const bool dense = in && (lam >= A.tau);
-    const bool sparse = in && !dense;
-    int kb = 0, kt = 0, nwin = 0;
-    if (dense && lam <= A.thresh) gz_window(lam, A.cap, kb, kt, nwin);
-    const uint64_t md = __ballot(dense), ms = __ballot(sparse), mb = __ballot(dense && nwin == 0);
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    double x = sparse ? lam : 0.0;
-    int xn = nwin;
+    const int nf = min(GZ_PREP_FG, F - f0);
+    double v_lam[GZ_PREP_FG], v_h[GZ_PREP_FG];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double y = __shfl_up(x, off, 64);
-        const int yn = __shfl_up(xn, off, 64);
-        if (lane >= off) {
-            x += y;
-            xn += yn;
+    for (int k = 0; k < GZ_PREP_FG; k++) {
+        const bool ok = b < A.B && k < nf;
+        v_lam[k] = ok ? A.number[b * F + f0 + k] : 0.0;
+        v_h[k] = ok ? A.h2fdf[b * F + f0 + k] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < GZ_PREP_FG; k++) {
+        if (k >= nf) break;
+        const int f = f0 + k;
+        if (k > 0) __syncthreads();   // the previous frequency's readers of s_w* are done
+        double lam = 0.0, h = 0.0;
+        bool in = b < A.B;
+        if (in) {
+            lam = v_lam[k];
+            h = v_h[k];
+            if (!(lam > 0.0)) in = false;
+            if (A.mode == HD_SEL_SORTED && skip_zero_h && h == 0.0) in = false;
+            // argmax: h <= 0 adds n*h = 0 and can never be selected
+            if (A.mode == HD_SEL_ARGMAX && !(h > 0.0)) in = false;
         }
-    }
-    if (lane == 63) {
-        s_wsum[wave] = x;
-        s_wn[wave] = xn;
-    }
-    if (lane == 0) {
-        s_wd[wave] = __popcll(md);
-        s_ws[wave] = __popcll(ms);
-        s_wb[wave] = __popcll(mb);
-    }
-    __syncthreads();
-    int od = 0, os = 0, on = 0, nd = 0, ns = 0, ntot = 0, nbig = 0;
-    double osum = 0.0;
-    for (int w = 0; w < GZ_TILE / 64; w++) {
-        if (w < wave) {
-            od += s_wd[w];
-            os += s_ws[w];
-            on += s_wn[w];
-            osum += s_wsum[w];
+        const bool dense = in && (lam >= A.tau);
+        const bool sparse = in && !dense;
+        int kb = 0, kt = 0, nwin = 0;
+        if (dense && lam <= A.thresh) gz_window(lam, A.cap, kb, kt, nwin);
+        const uint64_t md = __ballot(dense), ms = __ballot(sparse), mb = __ballot(dense && nwin == 0);
+        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        double x = sparse ? lam : 0.0;
+        int xn = nwin;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double y = __shfl_up(x, off, 64);
+            const int yn = __shfl_up(xn, off, 64);
+            if (lane >= off) {
+                x += y;
+                xn += yn;
+            }
         }
-        nd += s_wd[w];
-        ns += s_ws[w];
-        ntot += s_wn[w];
-        nbig += s_wb[w];
-    }
-    const int64_t ft = (int64_t)f * A.ntiles + tile;
-    const int64_t reg = ft * GZ_TILE;
-    const int64_t rreg = ft * GZ_REC;
-    const uint32_t rk = (A.mode == HD_SEL_SORTED && in) ? (uint32_t)A.rank[b] : 0u;
-    const uint64_t word = ((uint64_t)(uint32_t)b << 32) | rk;
-    if (dense) {
-        const int i = od + __popcll(md & lt);
-        DRec r;
-        r.lam = lam;
-        r.h = h;
-        r.word = word;
-        r.kb = kb;
-        r.kt = kt;
-        r.n = nwin;
-        r.p0 = (nwin > 0) ? gz_pmf_acc(lam, (double)kb) : 0.0;
-        r.off = on + xn - nwin;   // exclusive prefix of the table lengths
-        A.drec[rreg + i] = r;
-        A.rec_end[rreg + i] = r.off + nwin;
-        for (int k = 0; k < A.npar; k++)
-            A.d_par[(int64_t)k * F * A.ntiles * GZ_TILE + reg + i] = A.par[((int64_t)k * A.B + b) * F + f];
-    }
-    if (sparse) {
-        const int i = os + __popcll(ms & lt);
-        const double cum = osum + x;
-        A.s_cum[reg + i] = cum;
-        A.s_h[reg + i] = h;
-        A.s_word[reg + i] = word;
-        for (int k = 0; k < A.npar; k++)
-            A.sp_par[(int64_t)k * F * A.ntiles * GZ_TILE + reg + i] = A.par[((int64_t)k * A.B + b) * F + f];
-        if (i == ns - 1) {   // the last sparse bin writes the pseudo-record for the tile's point count
+        if (lane == 63) {
+            s_wsum[wave] = x;
+            s_wn[wave] = xn;
+        }
+        if (lane == 0) {
+            s_wd[wave] = __popcll(md);
+            s_ws[wave] = __popcll(ms);
+            s_wb[wave] = __popcll(mb);
+        }
+        __syncthreads();
+        int od = 0, os = 0, on = 0, nd = 0, ns = 0, ntot = 0, nbig = 0;
+        double osum = 0.0;
+        for (int w = 0; w < GZ_TILE / 64; w++) {
+            if (w < wave) {
+                od += s_wd[w];
+                os += s_ws[w];
+                on += s_wn[w];
+                osum += s_wsum[w];
+            }
+            nd += s_wd[w];
+            ns += s_ws[w];
+            ntot += s_wn[w];
+            nbig += s_wb[w];
+        }
+        const int64_t ft = (int64_t)f * A.ntiles + tile;
+        const int64_t reg = ft * GZ_TILE;
+        const int64_t rreg = ft * GZ_REC;
+        const uint32_t rk = (A.mode == HD_SEL_SORTED && in) ? (uint32_t)A.rank[b] : 0u;
+        const uint64_t word = ((uint64_t)(uint32_t)b << 32) | rk;
+        if (dense) {
+            const int i = od + __popcll(md & lt);
             DRec r;
-            r.lam = cum;
-            r.h = 0.0;
-            r.word = 0;
-            gz_window(cum, A.cap, r.kb, r.kt, r.n);
-            r.p0 = (r.n > 0) ? gz_pmf_acc(cum, (double)r.kb) : 0.0;
-            r.off = ntot;
-            A.drec[rreg + nd] = r;
-            A.rec_end[rreg + nd] = ntot + r.n;
+            r.lam = lam;
+            r.h = h;
+            r.word = word;
+            r.kb = kb;
+            r.kt = kt;
+            r.n = nwin;
+            r.p0 = (nwin > 0) ? gz_pmf_acc(lam, (double)kb) : 0.0;
+            r.off = on + xn - nwin;   // exclusive prefix of the table lengths
+            A.drec[rreg + i] = r;
+            A.rec_end[rreg + i] = r.off + nwin;
+            for (int k = 0; k < A.npar; k++)
+                A.d_par[(int64_t)k * F * A.ntiles * GZ_TILE + reg + i] = A.par[((int64_t)k * A.B + b) * F + f];
         }
-    }
-    if (tid == 0) {
-        A.cnt[ft * 3] = nd;
-        A.cnt[ft * 3 + 1] = ns;
-        A.cnt[ft * 3 + 2] = nbig;
-        double lam_sparse = 0.0;
-        for (int w = 0; w < GZ_TILE / 64; w++) lam_sparse += s_wsum[w];
-        A.tile_cost[ft] = (float)gz_tile_cost(nd, ntot, lam_sparse);
+        if (sparse) {
+            const int i = os + __popcll(ms & lt);
+            const double cum = osum + x;
+            A.s_cum[reg + i] = cum;
+            A.s_h[reg + i] = h;
+            A.s_word[reg + i] = word;
+            for (int k = 0; k < A.npar; k++)
+                A.sp_par[(int64_t)k * F * A.ntiles * GZ_TILE + reg + i] = A.par[((int64_t)k * A.B + b) * F + f];
+            if (i == ns - 1) {   // the last sparse bin writes the pseudo-record for the tile's point count
+                DRec r;
+                r.lam = cum;
+                r.h = 0.0;
+                r.word = 0;
+                gz_window(cum, A.cap, r.kb, r.kt, r.n);
+                r.p0 = (r.n > 0) ? gz_pmf_acc(cum, (double)r.kb) : 0.0;
+                r.off = ntot;
+                A.drec[rreg + nd] = r;
+                A.rec_end[rreg + nd] = ntot + r.n;
+            }
+        }
+        if (tid == 0) {
+            A.cnt[ft * 3] = nd;
+            A.cnt[ft * 3 + 1] = ns;
+            A.cnt[ft * 3 + 2] = nbig;
+            double lam_sparse = 0.0;
+            for (int w = 0; w < GZ_TILE / 64; w++) lam_sparse += s_wsum[w];
+            A.tile_cost[ft] = (float)gz_tile_cost(nd, ntot, lam_sparse);
+        }
     }
 }
 
@@ -1340,7 +1353,7 @@ static int realize_impl(const double* number, const double* h2fdf, const int64_t
         int rc = check_launch("k_gz_zero");
         if (rc) return rc;
     }
-    const int64_t prep_blocks = cdiv(A.ntiles, GZ_XCDS) * GZ_XCDS * (int64_t)F;
+    const int64_t prep_blocks = (int64_t)A.ntiles * cdiv(F, GZ_PREP_FG);
     if (prep_blocks > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: too many tiles x frequencies");
     if ((int64_t)A.nchunks * F > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: too many chunks x frequencies");
     hipLaunchKernelGGL(k_gz_prep, dim3((unsigned)prep_blocks), dim3(GZ_TILE), 0, st, A);

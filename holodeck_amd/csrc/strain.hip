@@ -14,13 +14,36 @@
 namespace hd {
 Cosmo make_cosmo(double om0, double h, const double* dc_tab, const double* dc_der, int n_tab);
 
+// K M_c (2 M_c)^(2/3) of mass bin mm, ratio bin qq: the factor of h_s that depends on the bin
+// only, so that h_s = g (f_r)^(2/3) / d_c with one cube root per output
+__device__ __forceinline__ double h2_mass_factor(const double* mtc, const double* mrc, int mm, int qq, double K) {
+    const double mt = mtc[mm], mr = mrc[qq];
+    const double mc = mt * pow(mr, 3.0 / 5.0) / pow(1 + mr, 6.0 / 5.0);
+    return K * mc * cbrt((2 * mc) * (2 * mc));
+}
+
 __global__ void __launch_bounds__(256) k_h2fdf(const double* __restrict__ mtc, int M, const double* __restrict__ mrc,
                                                int Q, int Z, const double* __restrict__ fc,
                                                const double* __restrict__ fcdf, int F,
                                                const double* __restrict__ rz, double K, Cosmo C,
                                                double* __restrict__ h2fdf) {
-    int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t nout = (int64_t)(M - 1) * (Q - 1) * (Z - 1) * F;
+    // outputs are (m, q, z, f) with f fastest; a block of 256 spans at most two (m, q) bins when a
+    // bin holds at least 256 outputs, and then their mass factors are computed once, in LDS
+    __shared__ double s_g[2];
+    const int64_t per_mq = (int64_t)(Z - 1) * F;
+    const int64_t o0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t mq0 = o0 / per_mq;
+    const int64_t nmq = (int64_t)(M - 1) * (Q - 1);
+    const bool shared_g = per_mq >= (int64_t)blockDim.x;
+    if (shared_g) {
+        if (threadIdx.x < 2 && mq0 + threadIdx.x < nmq) {
+            const int64_t mq = mq0 + threadIdx.x;
+            s_g[threadIdx.x] = h2_mass_factor(mtc, mrc, (int)(mq / (Q - 1)), (int)(mq % (Q - 1)), K);
+        }
+        __syncthreads();
+    }
+    int64_t o = o0 + threadIdx.x;
+    int64_t nout = nmq * (Z - 1) * F;
     if (o >= nout) return;
     int ff = (int)(o % F);
     int64_t t = o / F;
@@ -38,19 +61,13 @@ __global__ void __launch_bounds__(256) k_h2fdf(const double* __restrict__ mtc, i
     double b1 = 0.5 * (a01 + a11);
     double zmid = 0.5 * (b0 + b1);
 
-    double mt = mtc[mm], mr = mrc[qq];
-    double mc = mt * pow(mr, 3.0 / 5.0) / pow(1 + mr, 6.0 / 5.0);
+    const double g = shared_g ? s_g[t - mq0] : h2_mass_factor(mtc, mrc, mm, qq, K);
     double dc = (zmid > 0.0) ? cosmo_dcom(C, zmid) : __builtin_inf();
     double fr = fc[ff] * (1.0 + zmid);
-    double hs = K * mc * pow(2 * mc * fr, 2.0 / 3.0) / dc;
+    double hs = g * cbrt(fr * fr) / dc;   // K M_c (2 M_c f_r)^(2/3) / d_c
     h2fdf[o] = (hs * hs) * fcdf[ff];
 }
 
-// Per-bin parameters for the strain-weighted averages of the loudest walk
-// with parameters (single_sources.py:104-139): [mt, mr, rz] bin centres,
-// redz_final (midpointed, -1 where not positive), comoving distance [cm]
-// (+inf where redz_final <= 0), final separation [cm] (utils.py:1705-1724,
-// astropy G) and angular separation (utils.py:1897-1917).
 __global__ void __launch_bounds__(256) k_bin_params(const double* __restrict__ mtc, int M, const double* __restrict__ mrc,
                                                     int Q, const double* __restrict__ rzc, int Z,
                                                     const double* __restrict__ fc, int F, const double* __restrict__ rz,
