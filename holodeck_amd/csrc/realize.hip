@@ -20,6 +20,7 @@
 //     total - sum(selected h), which is exactly the reference's accounting
 //     (sum(n h) - max for ss_bg_hc; sum((n - taken) h) for the sorted walk,
 //     including its treatment of non-integer Normal draws).
+#include <type_traits>
 #include "common.hpp"
 #include "rng.hpp"
 #include "../../include/holohip.h"
@@ -175,10 +176,62 @@ __global__ void __launch_bounds__(512) k_realize_chunk(RealizeArgs A) {
     }
 }
 
+// Running list of the merges: LM slots (LM = L for the common L = 1, 5, 10; 16 slots holding
+// any L <= 16 otherwise).  S32: the sorted walk's 32-bit ranks (unique, no bins); else 64-bit
+// keys with the bin as tie-break (argmax).
+template <int LM, bool S32>
+struct MergeList {
+    using Key = typename std::conditional<S32, uint32_t, uint64_t>::type;
+    static constexpr Key NONE = S32 ? (Key)KEY32_NONE : (Key)KEY_NONE;
+    Key k[LM];
+    int32_t b[LM];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int i = 0; i < LM; i++) {
+            k[i] = NONE;
+            b[i] = 0x7fffffff;
+        }
+    }
+    __device__ __forceinline__ static bool better(Key ka, int32_t ba, Key kb, int32_t bb) {
+        return S32 ? (ka < kb) : ((ka < kb) || (ka == kb && ba < bb));
+    }
+    // (key, bin) of slot L - 1, the worst one kept
+    __device__ __forceinline__ void worst(int L, Key& wk, int32_t& wb) const {
+        if (LM != 16) {
+            wk = k[LM - 1];
+            wb = b[LM - 1];
+            return;
+        }
+        wk = NONE;
+        wb = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < LM; q++)
+            if (q == L - 1) {
+                wk = k[q];
+                wb = b[q];
+            }
+    }
+    __device__ __forceinline__ void insert(Key key, int32_t bin) {
+#pragma unroll
+        for (int i = 0; i < LM; i++) {
+            const bool sw = better(key, bin, k[i], b[i]);
+            const Key tk = k[i];
+            const int32_t tb = b[i];
+            k[i] = sw ? key : tk;
+            b[i] = sw ? bin : tb;
+            key = sw ? tk : key;
+            bin = sw ? tb : bin;
+        }
+    }
+};
+
 // Merge of chunk partials [c0, c1) of (f, r) into a running list, total and parameter totals.
-__device__ __forceinline__ void merge_chunks(const RealizeArgs& A, int f, int r, int c0, int c1, uint64_t (&lk)[LMAX],
-                                             int32_t (&lb)[LMAX], double& total, double (&ptot)[NPAR_MAX]) {
-    const int F = A.F, R = A.R, L = A.L, NC = A.nchunks;
+template <int LM, bool S32>
+__device__ __forceinline__ void merge_chunks(const RealizeArgs& A, int f, int r, int c0, int c1,
+                                             MergeList<LM, S32>& list, double& total, double (&ptot)[NPAR_MAX]) {
+    using Key = typename MergeList<LM, S32>::Key;
+    const int F = A.F, R = A.R, NC = A.nchunks;
+    const int L = (LM == 16) ? A.L : LM;
     const bool use_list = (A.mode != HD_SEL_NONE);
     for (int c = c0; c < c1; c++) {
         int64_t pr = ((int64_t)f * NC + c) * R + r;
@@ -188,34 +241,27 @@ __device__ __forceinline__ void merge_chunks(const RealizeArgs& A, int f, int r,
             // the chunk's whole list first: every load issued before any is used (indices clamped
             // to slot L-1, no branch), so a chunk costs one load latency instead of one per slot
             const int64_t o0 = ((int64_t)f * NC + c) * L * (int64_t)R + r;
-            uint64_t ck[LMAX];
-            int32_t cb[LMAX];
-            if (A.mode == HD_SEL_SORTED) {
-                const uint32_t* pk = (const uint32_t*)A.p_key;
+            Key ck[LM];
+            int32_t cb[LM];
 #pragma unroll
-                for (int j = 0; j < LMAX; j++) {
-                    const uint32_t k = pk[o0 + (int64_t)min(j, L - 1) * R];
-                    ck[j] = (k == KEY32_NONE) ? KEY_NONE : (uint64_t)k;
+            for (int j = 0; j < LM; j++) {
+                const int64_t o = o0 + (int64_t)min(j, L - 1) * R;
+                if (S32) {
+                    ck[j] = (Key)((const uint32_t*)A.p_key)[o];
                     cb[j] = 0;   // ranks are unique: no tie-break
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < LMAX; j++) {
-                    const int64_t o = o0 + (int64_t)min(j, L - 1) * R;
-                    ck[j] = A.p_key[o];
+                } else {
+                    ck[j] = (Key)A.p_key[o];
                     cb[j] = A.p_bin[o];
                 }
             }
 #pragma unroll
-            for (int j = 0; j < LMAX; j++) {
-                if (j >= L || ck[j] == KEY_NONE) break;
-                uint64_t wk = KEY_NONE;
-                int32_t wb = 0x7fffffff;
-#pragma unroll
-                for (int q = 0; q < LMAX; q++)
-                    if (q == L - 1) { wk = lk[q]; wb = lb[q]; }
-                if (!better(ck[j], cb[j], wk, wb)) break;
-                list_insert(lk, lb, ck[j], cb[j]);
+            for (int j = 0; j < LM; j++) {
+                if (j >= L || ck[j] == MergeList<LM, S32>::NONE) break;
+                Key wk;
+                int32_t wb;
+                list.worst(L, wk, wb);
+                if (!MergeList<LM, S32>::better(ck[j], cb[j], wk, wb)) break;
+                list.insert(ck[j], cb[j]);
             }
         }
     }
@@ -226,79 +272,73 @@ constexpr int MERGE_GROUPS = 8;
 // First merge stage: thread per (f, group, r) merges the group's chunks and writes a partial of
 // the same layout with G "chunks", so that the final merge reads G partials instead of NC (the
 // final merge alone has one thread per (f, r): too few waves to hide its load latency).
+template <int LM, bool S32>
 __global__ void __launch_bounds__(256) k_realize_group(RealizeArgs A, int G, double* __restrict__ q_total,
                                                        double* __restrict__ q_ptot, uint64_t* __restrict__ q_key,
                                                        int32_t* __restrict__ q_bin) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int F = A.F, R = A.R, L = A.L, NC = A.nchunks;
+    const int F = A.F, R = A.R, NC = A.nchunks;
+    const int L = (LM == 16) ? A.L : LM;
     if (t >= (int64_t)F * G * R) return;
     const int r = (int)(t % R), g = (int)((t / R) % G), f = (int)(t / ((int64_t)G * R));
     const int CG = (NC + G - 1) / G;
     const int c0 = min(g * CG, NC), c1 = min(c0 + CG, NC);
-    uint64_t lk[LMAX];
-    int32_t lb[LMAX];
-#pragma unroll
-    for (int i = 0; i < LMAX; i++) {
-        lk[i] = KEY_NONE;
-        lb[i] = 0x7fffffff;
-    }
+    MergeList<LM, S32> list;
+    list.init();
     double total = 0.0;
     double ptot[NPAR_MAX];
 #pragma unroll
     for (int k = 0; k < NPAR_MAX; k++) ptot[k] = 0.0;
-    merge_chunks(A, f, r, c0, c1, lk, lb, total, ptot);
+    merge_chunks<LM, S32>(A, f, r, c0, c1, list, total, ptot);
     const int64_t pr = ((int64_t)f * G + g) * R + r;
     q_total[pr] = total;
     for (int k = 0; k < A.npar; k++) q_ptot[(int64_t)k * F * G * R + pr] = ptot[k];
     if (A.mode != HD_SEL_NONE) {
 #pragma unroll
-        for (int j = 0; j < LMAX; j++)
+        for (int j = 0; j < LM; j++)
             if (j < L) {
                 const int64_t o = (((int64_t)f * G + g) * L + j) * R + r;
-                if (A.mode == HD_SEL_SORTED) {
-                    ((uint32_t*)q_key)[o] = (lk[j] == KEY_NONE) ? KEY32_NONE : (uint32_t)lk[j];
+                if (S32) {
+                    ((uint32_t*)q_key)[o] = (uint32_t)list.k[j];
                 } else {
-                    q_key[o] = lk[j];
-                    q_bin[o] = lb[j];
+                    q_key[o] = (uint64_t)list.k[j];
+                    q_bin[o] = list.b[j];
                 }
             }
     }
 }
 
+template <int LM, bool S32>
 __global__ void __launch_bounds__(256) k_realize_merge(RealizeArgs A, double* __restrict__ hc2ss,
                                                        int64_t* __restrict__ ssidx, double* __restrict__ hc2bg,
                                                        double* __restrict__ bgpar) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int F = A.F, R = A.R, L = A.L, NC = A.nchunks;
+    const int F = A.F, R = A.R, NC = A.nchunks;
+    const int L = (LM == 16) ? A.L : LM;
     if (t >= (int64_t)F * R) return;
     const int f = (int)(t / R), r = (int)(t % R);
     const bool use_list = (A.mode != HD_SEL_NONE);
-    uint64_t lk[LMAX];
-    int32_t lb[LMAX];
-#pragma unroll
-    for (int i = 0; i < LMAX; i++) {
-        lk[i] = KEY_NONE;
-        lb[i] = 0x7fffffff;
-    }
+    MergeList<LM, S32> list;
+    list.init();
     double total = 0.0;
     double ptot[NPAR_MAX];
 #pragma unroll
     for (int k = 0; k < NPAR_MAX; k++) ptot[k] = 0.0;
-    merge_chunks(A, f, r, 0, NC, lk, lb, total, ptot);
+    merge_chunks<LM, S32>(A, f, r, 0, NC, list, total, ptot);
     double sel = 0.0;
     double psel[NPAR_MAX];
 #pragma unroll
     for (int k = 0; k < NPAR_MAX; k++) psel[k] = 0.0;
     if (use_list) {
 #pragma unroll
-        for (int j = 0; j < LMAX; j++) {
+        for (int j = 0; j < LM; j++) {
             if (j >= L) continue;
             int64_t o = ((int64_t)f * R + r) * L + j;
-            if (lk[j] == KEY_NONE) {
+            if (list.k[j] == MergeList<LM, S32>::NONE) {
                 hc2ss[o] = 0.0;
                 ssidx[o] = -1;
             } else {
-                const int64_t b = (A.mode == HD_SEL_SORTED) ? A.order[lk[j]] : (int64_t)lb[j];
+                const int64_t b = S32 ? A.order[list.k[j]] : (int64_t)list.b[j];
                 double h = A.h2fdf[b * F + f];
                 hc2ss[o] = h;
                 ssidx[o] = b;
@@ -310,7 +350,7 @@ __global__ void __launch_bounds__(256) k_realize_merge(RealizeArgs A, double* __
     double bg = total - sel;
     hc2bg[(int64_t)f * R + r] = bg;
     if (bgpar) {
-        bool none = use_list && (lk[0] == KEY_NONE) && (A.mode == HD_SEL_ARGMAX);
+        bool none = use_list && (list.k[0] == MergeList<LM, S32>::NONE) && (A.mode == HD_SEL_ARGMAX);
         for (int k = 0; k < A.npar; k++) {
             double v = none ? __builtin_nan("") : (ptot[k] - psel[k]) / bg;
             bgpar[((int64_t)k * F + f) * R + r] = v;
@@ -408,6 +448,34 @@ size_t realize_group_ws(int F, int R, int L, int npar) {
            4 * 256;
 }
 
+// the two merge stages for list capacity LM and key width (S32: the sorted walk's ranks)
+template <int LM, bool S32>
+static int merge_launch_k(RealizeArgs A, void* group_ws, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
+                          hipStream_t st) {
+    const int F = A.F, R = A.R, L = A.L, npar = A.npar;
+    if (group_ws && A.nchunks > 2 * MERGE_GROUPS) {
+        const int G = MERGE_GROUPS;
+        const size_t n = (size_t)F * G * R;
+        char* w = (char*)(((uintptr_t)group_ws + 255) & ~(uintptr_t)255);
+        double* q_total = (double*)w;         w += n * sizeof(double);
+        double* q_ptot = (double*)w;          w += n * sizeof(double) * (size_t)npar;
+        uint64_t* q_key = (uint64_t*)w;       w += n * sizeof(uint64_t) * (size_t)L;
+        int32_t* q_bin = (int32_t*)w;
+        hipLaunchKernelGGL((k_realize_group<LM, S32>), dim3((unsigned)cdiv((int64_t)n, 256)), dim3(256), 0, st, A, G,
+                           q_total, q_ptot, q_key, q_bin);
+        int rc = check_launch("k_realize_group");
+        if (rc) return rc;
+        A.nchunks = G;
+        A.p_total = q_total;
+        A.p_ptot = q_ptot;
+        A.p_key = q_key;
+        A.p_bin = q_bin;
+    }
+    hipLaunchKernelGGL((k_realize_merge<LM, S32>), dim3((unsigned)cdiv((int64_t)F * R, 256)), dim3(256), 0, st, A, hc2ss,
+                       ssidx, hc2bg, bgpar);
+    return check_launch("k_realize_merge");
+}
+
 int realize_merge_launch(const double* h2fdf, const double* par, const int64_t* order, int64_t B, int F, int R, int L,
                          int mode, int npar, int nchunks, const double* p_total, const double* p_ptot, const uint64_t* p_key,
                          const int32_t* p_bin, double* hc2ss, int64_t* ssidx, double* hc2bg, double* bgpar,
@@ -430,27 +498,19 @@ int realize_merge_launch(const double* h2fdf, const double* par, const int64_t* 
     A.p_ptot = const_cast<double*>(p_ptot);
     A.p_key = const_cast<uint64_t*>(p_key);
     A.p_bin = const_cast<int32_t*>(p_bin);
-    if (group_ws && nchunks > 2 * MERGE_GROUPS) {
-        const int G = MERGE_GROUPS;
-        const size_t n = (size_t)F * G * R;
-        char* w = (char*)(((uintptr_t)group_ws + 255) & ~(uintptr_t)255);
-        double* q_total = (double*)w;         w += n * sizeof(double);
-        double* q_ptot = (double*)w;          w += n * sizeof(double) * (size_t)npar;
-        uint64_t* q_key = (uint64_t*)w;       w += n * sizeof(uint64_t) * (size_t)L;
-        int32_t* q_bin = (int32_t*)w;
-        hipLaunchKernelGGL(k_realize_group, dim3((unsigned)cdiv((int64_t)n, 256)), dim3(256), 0, st, A, G, q_total,
-                           q_ptot, q_key, q_bin);
-        int rc = check_launch("k_realize_group");
-        if (rc) return rc;
-        A.nchunks = G;
-        A.p_total = q_total;
-        A.p_ptot = q_ptot;
-        A.p_key = q_key;
-        A.p_bin = q_bin;
+    if (mode == HD_SEL_SORTED) {
+        if (L == 1) return merge_launch_k<1, true>(A, group_ws, hc2ss, ssidx, hc2bg, bgpar, st);
+        if (L == 5) return merge_launch_k<5, true>(A, group_ws, hc2ss, ssidx, hc2bg, bgpar, st);
+        if (L == 10) return merge_launch_k<10, true>(A, group_ws, hc2ss, ssidx, hc2bg, bgpar, st);
+        return merge_launch_k<16, true>(A, group_ws, hc2ss, ssidx, hc2bg, bgpar, st);
     }
-    hipLaunchKernelGGL(k_realize_merge, dim3((unsigned)cdiv((int64_t)F * R, 256)), dim3(256), 0, st, A, hc2ss, ssidx,
-                       hc2bg, bgpar);
-    return check_launch("k_realize_merge");
+    if (mode == HD_SEL_ARGMAX) {
+        if (L == 1) return merge_launch_k<1, false>(A, group_ws, hc2ss, ssidx, hc2bg, bgpar, st);
+        if (L == 5) return merge_launch_k<5, false>(A, group_ws, hc2ss, ssidx, hc2bg, bgpar, st);
+        if (L == 10) return merge_launch_k<10, false>(A, group_ws, hc2ss, ssidx, hc2bg, bgpar, st);
+        return merge_launch_k<16, false>(A, group_ws, hc2ss, ssidx, hc2bg, bgpar, st);
+    }
+    return merge_launch_k<1, false>(A, group_ws, hc2ss, ssidx, hc2bg, bgpar, st);   // no lists
 }
 
 int rank_from_order_launch(const int64_t* order, int64_t B, int32_t* rank, hipStream_t st) {

@@ -216,6 +216,8 @@ constexpr int GZ_PREP_FG = 8;   // frequencies per prep workgroup
 __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
     __shared__ int s_wd[GZ_TILE / 64], s_ws[GZ_TILE / 64], s_wn[GZ_TILE / 64], s_wb[GZ_TILE / 64];
     __shared__ double s_wsum[GZ_TILE / 64];
+    __shared__ double s_plam[GZ_TILE];   // dense records' lambda and first count, compacted: the
+    __shared__ int s_pkb[GZ_TILE];       // pmf anchors are evaluated with full waves
     // workgroup per (tile, group of GZ_PREP_FG frequencies): each thread reads its bin's
     // GZ_PREP_FG consecutive values of number and h2fdf (whole cache lines, instead of one
     // 8-byte word per line and frequency), then the frequencies are classified in turn
@@ -304,8 +306,10 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
             r.kb = kb;
             r.kt = kt;
             r.n = nwin;
-            r.p0 = (nwin > 0) ? gz_pmf_acc(lam, (double)kb) : 0.0;
+            r.p0 = 0.0;               // pmf anchor p(kb): below, with the records compacted
             r.off = on + xn - nwin;   // exclusive prefix of the table lengths
+            s_plam[i] = lam;
+            s_pkb[i] = (nwin > 0) ? kb : -1;
             A.drec[rreg + i] = r;
             A.rec_end[rreg + i] = r.off + nwin;
             for (int k = 0; k < A.npar; k++)
@@ -339,6 +343,8 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
             for (int w = 0; w < GZ_TILE / 64; w++) lam_sparse += s_wsum[w];
             A.tile_cost[ft] = (float)gz_tile_cost(nd, ntot, lam_sparse);
         }
+        __syncthreads();   // the compacted (lambda, kb) of every dense record are in LDS
+        if (tid < nd && s_pkb[tid] >= 0) A.drec[rreg + tid].p0 = gz_pmf_acc(s_plam[tid], (double)s_pkb[tid]);
     }
 }
 
