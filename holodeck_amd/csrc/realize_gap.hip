@@ -221,9 +221,15 @@ __global__ void __launch_bounds__(GZ_TILE) k_gz_prep(GapArgs A) {
     // workgroup per (tile, group of GZ_PREP_FG frequencies): each thread reads its bin's
     // GZ_PREP_FG consecutive values of number and h2fdf (whole cache lines, instead of one
     // 8-byte word per line and frequency), then the frequencies are classified in turn
+    // XCD-aware order: workgroup w runs on XCD w % 8, and the workgroups of one XCD take the
+    // frequency groups of one tile in turn, so that the 128-byte lines a tile's rows share
+    // between groups are fetched once into that XCD's L2
     const int ngroups = (A.F + GZ_PREP_FG - 1) / GZ_PREP_FG;
-    const int tile = blockIdx.x / ngroups;
-    const int f0 = (blockIdx.x - tile * ngroups) * GZ_PREP_FG;
+    const int w = blockIdx.x;
+    const int k8 = w / GZ_XCDS;
+    const int tile = (k8 / ngroups) * GZ_XCDS + (w % GZ_XCDS);
+    if (tile >= A.ntiles) return;
+    const int f0 = (k8 % ngroups) * GZ_PREP_FG;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int F = A.F;
     const int64_t b = (int64_t)tile * GZ_TILE + tid;
@@ -1359,7 +1365,7 @@ static int realize_impl(const double* number, const double* h2fdf, const int64_t
         int rc = check_launch("k_gz_zero");
         if (rc) return rc;
     }
-    const int64_t prep_blocks = (int64_t)A.ntiles * cdiv(F, GZ_PREP_FG);
+    const int64_t prep_blocks = cdiv(A.ntiles, GZ_XCDS) * GZ_XCDS * cdiv(F, GZ_PREP_FG);
     if (prep_blocks > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: too many tiles x frequencies");
     if ((int64_t)A.nchunks * F > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_realize: too many chunks x frequencies");
     hipLaunchKernelGGL(k_gz_prep, dim3((unsigned)prep_blocks), dim3(GZ_TILE), 0, st, A);
