@@ -58,6 +58,48 @@ __device__ double lifetime_wave(const EvoConsts& K, const LifeArgs& a, double no
     return wave_sum(part) - a.target;
 }
 
+// The factors of the hardening rate that do not depend on the normalisation, at this lane's
+// step boundaries, computed once per (M, Q) instead of at every root-finder iteration: the
+// separation, (1 + x)^(g_in - g_out), x^(g_in - 1) with x = a / r_char, the GW rate, and the
+// step's 2 (a_right - a_left).  The rate is then ((-norm) * p1) / p2 + gw, the same operations in
+// the same order as hard_func_2pwl_gw, so every lifetime is bit for bit the one computed from
+// scratch.  Up to LIFE_PTS - 1 steps per lane (nsteps <= 64 (LIFE_PTS - 1)).
+constexpr int LIFE_PTS = 9;
+struct LifePre {
+    double p1[LIFE_PTS], p2[LIFE_PTS], gw[LIFE_PTS], num[LIFE_PTS];
+};
+
+__device__ __forceinline__ void lifetime_pre(const EvoConsts& K, const LifeArgs& a, LifePre& P) {
+    const int npts = (a.s_begin < a.s_end) ? a.s_end - a.s_begin + 1 : 0;
+    double x = a.x_lane0, sepa_prev = 0.0;
+#pragma unroll
+    for (int i = 0; i < LIFE_PTS; i++) {
+        if (i > 0) x -= a.dx;
+        const double sepa = pow(10.0, x);
+        const double xx = sepa / a.rchar;
+        const bool on = i < npts;
+        P.p1[i] = on ? pow(1.0 + xx, -a.g_out + a.g_in) : 1.0;
+        P.p2[i] = on ? pow(xx, a.g_in - 1.0) : 1.0;
+        P.gw[i] = on ? hard_gw(K, a.mt, a.mr, sepa) : 0.0;
+        P.num[i] = (on && i > 0) ? 2.0 * (sepa - sepa_prev) : 0.0;
+        sepa_prev = sepa;
+    }
+}
+
+__device__ double lifetime_wave_pre(const LifeArgs& a, const LifePre& P, double norm_log10) {
+    const double norm = pow(10.0, norm_log10);
+    const int nst = (a.s_begin < a.s_end) ? a.s_end - a.s_begin : 0;
+    double part = 0.0;
+    double dadt_left = -norm * P.p1[0] / P.p2[0] + P.gw[0];
+#pragma unroll
+    for (int i = 1; i < LIFE_PTS; i++) {
+        const double dadt_right = -norm * P.p1[i] / P.p2[i] + P.gw[i];
+        if (i <= nst) part += P.num[i] / (dadt_left + dadt_right);
+        dadt_left = dadt_right;
+    }
+    return wave_sum(part) - a.target;
+}
+
 __global__ void __launch_bounds__(256) k_find_2pwl_norm(EvoConsts K, double target_time, const double* __restrict__ mtot,
                                                         const double* __restrict__ mrat, int64_t n, double sepa_init,
                                                         double rchar, double g_in, double g_out, int nsteps,
@@ -83,11 +125,17 @@ __global__ void __launch_bounds__(256) k_find_2pwl_norm(EvoConsts K, double targ
     for (int s = 0; s < a.s_begin; s++) x -= a.dx;
     a.x_lane0 = x;
 
+    // the normalisation-free factors of every step boundary once (registers), when they fit
+    const bool pre = chunk <= LIFE_PTS - 1;
+    LifePre P;
+    if (pre) lifetime_pre(K, a, P);
+    auto life = [&](double v) { return pre ? lifetime_wave_pre(a, P, v) : lifetime_wave(K, a, v); };
+
     const double xtol = 1e-3, rtol = 1e-5;
     const int maxiter = 100;
     double xpre = -20.0, xcur = 20.0, xblk = 0.0, fblk = 0.0, spre = 0.0, scur = 0.0;
-    double fpre = lifetime_wave(K, a, xpre);
-    double fcur = lifetime_wave(K, a, xcur);
+    double fpre = life(xpre);
+    double fcur = life(xcur);
     double res = xcur;
     bool done = false;
     if (fpre == 0) { res = xpre; done = true; }
@@ -130,7 +178,7 @@ __global__ void __launch_bounds__(256) k_find_2pwl_norm(EvoConsts K, double targ
         fpre = fcur;
         if (fabs(scur) > delta) xcur += scur;
         else xcur += (sbis > 0 ? delta : -delta);
-        fcur = lifetime_wave(K, a, xcur);
+        fcur = life(xcur);
         res = xcur;
     }
     if (lane == 0) {
