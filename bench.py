@@ -264,13 +264,13 @@ def main():
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=None,
                     help=f"CPU-baseline processes (default: host cores, at most {CPU_PROCS_MAX})")
     ap.add_argument("--nreals", type=int, default=500)
-    ap.add_argument("--streams", type=int, default=3,
+    ap.add_argument("--streams", type=int, default=2,
                     help="independent spectra in flight on this many HIP streams (consecutive steps alternate)")
     ap.add_argument("--roofline-steps", type=int, default=3,
                     help="single-stream steps after the timed region that time the realization launch")
@@ -309,12 +309,18 @@ def main():
         hard = configs.make_hard(spec, sam)
         return sam.gwb_device(fe, hard, R, L, seed=seed)
 
-    # warm-up on the default stream: builds the cached scatter plan and device tables that the
-    # streams then only read
-    for w in range(args.warmup):
-        out = step(1000 + w)
-    torch.cuda.synchronize()
+    # warm-up: the first step on the default stream builds the cached scatter plan and device
+    # tables that the streams then only read; the rest alternate over the streams as the timed
+    # steps do, so that each stream's memory pool holds its workspaces before the timing starts
     streams = [torch.cuda.Stream() for _ in range(max(1, args.streams))]   # default priority
+    for w in range(args.warmup):
+        if w == 0:
+            out = step(1000 + w)
+            torch.cuda.synchronize()
+            continue
+        with torch.cuda.stream(streams[(w - 1) % len(streams)]):
+            out = step(1000 + w)
+    torch.cuda.synchronize()
     engine.EVENTS = {}
     _lib.timing_enable(True)
     torch.cuda.synchronize()
@@ -333,6 +339,7 @@ def main():
         # realization launch is timed on its own over a few steps on one stream (after, and
         # outside, the timed steps); enabling clears the records of the timed steps
         _lib.timing_enable(True)
+        out_r = None
         for s in range(args.roofline_steps):
             out_r = step(2000 + s)
         torch.cuda.synchronize()
