@@ -834,6 +834,77 @@ __global__ void __launch_bounds__(256) k_ct_bary_nearest(const double* __restric
     if (in) q_nearest[g] = bi;
 }
 
+// C = A B (row-major, FP64) for the large products: 128 x 128 output tiles, 256 threads with
+// 8 x 8 outputs each (64 accumulators: four FMAs per LDS word read, against two for the 64 x 64
+// tiles above), K in slices of 8 staged through LDS (A transposed), the next slice's global
+// loads issued before the current slice is multiplied.  A thread's 8 columns are two groups of
+// 4 at tx*4 and 64 + tx*4, its 8 rows ty*4 and 64 + ty*4, so that a wave's LDS reads and global
+// stores are contiguous.
+constexpr int GT2 = 128, GK2 = 8;
+__global__ void __launch_bounds__(256) k_gemm_f64_big(const double* __restrict__ A, const double* __restrict__ B,
+                                                      double* __restrict__ C, int m, int n, int k) {
+    __shared__ double sA[GK2][GT2];
+    __shared__ double sB[GK2][GT2];
+    const int tid = threadIdx.x;
+    const int tx = tid % 16, ty = tid / 16;
+    const int row0 = blockIdx.y * GT2, col0 = blockIdx.x * GT2;
+    double acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[i][j] = 0.0;
+    // staging: A tile 128 x 8 (4 words per thread: row ar + 32 q, column ak), B tile 8 x 128
+    // (4 words per thread: row bk, column bc + 32 q)
+    const int ar = tid / GK2, ak = tid % GK2;
+    const int bk = tid / 32, bc = tid % 32;
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int gr = row0 + ar + 32 * q, gk = k0 + ak;
+            ra[q] = (gr < m && gk < k) ? A[(int64_t)gr * k + gk] : 0.0;
+            const int gk2 = k0 + bk, gc = col0 + bc + 32 * q;
+            rb[q] = (gk2 < k && gc < n) ? B[(int64_t)gk2 * n + gc] : 0.0;
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < k; k0 += GK2) {
+        __syncthreads();   // the previous slice's readers are done
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            sA[ak][ar + 32 * q] = ra[q];
+            sB[bk][bc + 32 * q] = rb[q];
+        }
+        __syncthreads();
+        if (k0 + GK2 < k) load(k0 + GK2);   // in flight while this slice is multiplied
+#pragma unroll
+        for (int kk = 0; kk < GK2; kk++) {
+            double a[8], b[8];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                a[i] = sA[kk][ty * 4 + i];
+                a[4 + i] = sA[kk][64 + ty * 4 + i];
+                b[i] = sB[kk][tx * 4 + i];
+                b[4 + i] = sB[kk][64 + tx * 4 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int r = row0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + i - 4);
+        if (r >= m) continue;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = col0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + j - 4);
+            if (c < n) C[(int64_t)r * n + c] = acc[i][j];
+        }
+    }
+}
+
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace hd
@@ -842,6 +913,11 @@ using namespace hd;
 
 extern "C" int hd_gemm_f64(const double* A, const double* B, double* C, int m, int n, int k, void* stream) {
     if (m < 1 || n < 1 || k < 1 || !A || !B || !C) return set_error(HD_ERR_ARG, "hd_gemm_f64: bad arguments");
+    if ((int64_t)m * n >= (int64_t)64 * GT2 * GT2) {   // enough 128 x 128 tiles to fill the GPU
+        dim3 grid((unsigned)cdiv(n, GT2), (unsigned)cdiv(m, GT2));
+        hipLaunchKernelGGL(k_gemm_f64_big, grid, dim3(256), 0, (hipStream_t)stream, A, B, C, m, n, k);
+        return check_launch("k_gemm_f64_big");
+    }
     dim3 grid((unsigned)cdiv(n, GT), (unsigned)cdiv(m, GT));
     hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, (hipStream_t)stream, A, B, C, m, n, k);
     return check_launch("k_gemm_f64");
