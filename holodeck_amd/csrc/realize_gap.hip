@@ -898,6 +898,32 @@ __device__ __forceinline__ GzTab gz_tab(const uint32_t* tab, const BRec* b, bool
     return t;
 }
 
+// The table of a record read from global memory through the constant address space (the
+// fields are wave-uniform: scalar loads); off is relative to the batch's first table.
+typedef const __attribute__((address_space(4))) DRec* gz_cdrec;
+__device__ __forceinline__ DRec gz_load_rec(gz_cdrec p) {
+    DRec d;
+    d.lam = p->lam;
+    d.h = p->h;
+    d.word = p->word;
+    d.p0 = p->p0;
+    d.kb = p->kb;
+    d.n = p->n;
+    d.off = p->off;
+    d.kt = p->kt;
+    return d;
+}
+__device__ __forceinline__ GzTab gz_tab_rec(const uint32_t* tab, const DRec& d, int base, const BRec* staged,
+                                            bool present) {
+    GzTab t;
+    t.n = present ? d.n : 0;
+    t.hi = (const gz_lds_u32*)(tab + 2 * (d.off - base));
+    t.kb = d.kb;
+    t.kt = d.kt;
+    t.rec = staged;
+    return t;
+}
+
 // Counts without a table: Normal above the threshold, inversion below 10,
 // PTRS otherwise; stream (a, f, r, tag).
 __device__ __forceinline__ double gz_fallback(uint32_t a, uint32_t f, uint32_t r, uint32_t tag, uint64_t seed,
@@ -1045,19 +1071,24 @@ __global__ void __launch_bounds__(GZ_MAXBS, 6) k_gz_draw(GapArgs A) {
                     // dense records of the batch (one control path, so the running list needs no
                     // copies between branches), then the tile's sparse pseudo-record if it is here
                     // (records in pairs, their searches interleaved; n = 0: no table, drawn by k_gz_big)
+                    // the records' fields are the same for every lane: read through the constant
+                    // address space (scalar loads into SGPRs, no VALU moves); the staged LDS copy
+                    // serves the builds and the rare tail walk
+                    const gz_cdrec cdr = (gz_cdrec)(A.drec + rreg);
                     const int cd = min(c1, nd);
                     for (int i = c; i < cd; i += 2) {
-                        const BRec* pa = brec + (i - c);
                         const bool hb = i + 1 < cd;
-                        const BRec* pb = brec + (hb ? i + 1 - c : i - c);
-                        const GzTab ta = gz_tab(tab, pa, true), tb = gz_tab(tab, pb, hb);
+                        const DRec ra = gz_load_rec(cdr + i);
+                        const DRec rb = gz_load_rec(cdr + (hb ? i + 1 : i));
+                        const GzTab ta = gz_tab_rec(tab, ra, base, brec + (i - c), true);
+                        const GzTab tb = gz_tab_rec(tab, rb, base, brec + (hb ? i + 1 - c : i - c), hb);
                         int ka, kb;
                         gz_draw2(ta, tb, rs, ka, kb);
-                        if (ta.n > 0) acc.add((double)ka, pa->h, pa->word, A.d_par + reg + i, pstride, A.npar, L);
-                        if (tb.n > 0) acc.add((double)kb, pb->h, pb->word, A.d_par + reg + i + 1, pstride, A.npar, L);
+                        if (ta.n > 0) acc.add((double)ka, ra.h, ra.word, A.d_par + reg + i, pstride, A.npar, L);
+                        if (tb.n > 0) acc.add((double)kb, rb.h, rb.word, A.d_par + reg + i + 1, pstride, A.npar, L);
                         if (REC) {
-                            if (ta.n > 0) gz_record(A, pa->word, f, r, (double)ka, false);
-                            if (tb.n > 0) gz_record(A, pb->word, f, r, (double)kb, false);
+                            if (ta.n > 0) gz_record(A, ra.word, f, r, (double)ka, false);
+                            if (tb.n > 0) gz_record(A, rb.word, f, r, (double)kb, false);
                         }
                     }
                     for (int i = max(c, nd); i < c1; i++) {
