@@ -534,19 +534,21 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
 // One thread per output (m,q,z,f); corner order and operation order follow
 // the reference loop.
 // ---------------------------------------------------------------------------
+// block b: (mass, ratio) bin b / nchunk, chunk b % nchunk of its 256-output chunks of (redshift
+// bin, frequency) -- the chunks of a bin consecutive, as the corner rows they share; a thread
+// decodes its output with one 32-bit division (the flat decode took three 64-bit ones)
 __global__ void __launch_bounds__(256) k_integrate_3dx1d(const double* __restrict__ lm, int M, const double* __restrict__ mr,
                                                          int Q, const double* __restrict__ rz, int Z,
                                                          const double* __restrict__ dlnf, int F,
                                                          const double* __restrict__ dnum, double* __restrict__ numb) {
-    int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t nout = (int64_t)(M - 1) * (Q - 1) * (Z - 1) * F;
-    if (o >= nout) return;
-    int ff = (int)(o % F);
-    int64_t t = o / F;
-    int zz = (int)(t % (Z - 1));
-    t /= (Z - 1);
-    int qq = (int)(t % (Q - 1));
-    int mm = (int)(t / (Q - 1));
+    const uint32_t per_mq = (uint32_t)(Z - 1) * (uint32_t)F;
+    const uint32_t nchunk = (per_mq + blockDim.x - 1) / blockDim.x;
+    const int mq = (int)(blockIdx.x / nchunk);   // uniform: the block's chunks of one bin are consecutive
+    const int mm = mq / (Q - 1), qq = mq - mm * (Q - 1);
+    const uint32_t loc = (blockIdx.x - (uint32_t)mq * nchunk) * blockDim.x + threadIdx.x;
+    if (loc >= per_mq) return;
+    const int zz = (int)(loc / (uint32_t)F), ff = (int)(loc - (uint32_t)zz * (uint32_t)F);
+    const int64_t o = (int64_t)mq * per_mq + loc;
     double dm = lm[mm + 1] - lm[mm];
     double dmdq = dm * (mr[qq + 1] - mr[qq]);
     double dmdqdz = dmdq * (rz[zz + 1] - rz[zz]);
@@ -642,8 +644,10 @@ extern "C" int hd_lifetime_2pwl(const double* norm_log10, const double* mtot, co
 extern "C" int hd_integrate_3dx1d(const double* log10_mtot, int M, const double* mrat, int Q, const double* redz, int Z,
                                   const double* dln_freq, int F, const double* dnum, double* numb, void* stream) {
     if (M < 2 || Q < 2 || Z < 2 || F < 1) return set_error(HD_ERR_ARG, "hd_integrate_3dx1d: bad sizes");
-    int64_t n = (int64_t)(M - 1) * (Q - 1) * (Z - 1) * F;
-    hipLaunchKernelGGL(k_integrate_3dx1d, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, log10_mtot,
-                       M, mrat, Q, redz, Z, dln_freq, F, dnum, numb);
+    const int64_t per_mq = (int64_t)(Z - 1) * F;
+    const int64_t nblk = (int64_t)(M - 1) * (Q - 1) * cdiv(per_mq, 256);
+    if (per_mq > 0xffffffffll || nblk > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_integrate_3dx1d: grid too large");
+    hipLaunchKernelGGL(k_integrate_3dx1d, dim3((unsigned)nblk), dim3(256), 0,
+                       (hipStream_t)stream, log10_mtot, M, mrat, Q, redz, Z, dln_freq, F, dnum, numb);
     return check_launch("k_integrate_3dx1d");
 }

@@ -22,35 +22,25 @@ __device__ __forceinline__ double h2_mass_factor(const double* mtc, const double
     return K * mc * cbrt((2 * mc) * (2 * mc));
 }
 
+// block b: (mass, ratio) bin b / nchunk, chunk b % nchunk of its 256-output chunks of (redshift
+// bin, frequency), the chunks of a bin consecutive; one 32-bit division per thread
 __global__ void __launch_bounds__(256) k_h2fdf(const double* __restrict__ mtc, int M, const double* __restrict__ mrc,
                                                int Q, int Z, const double* __restrict__ fc,
                                                const double* __restrict__ fcdf, int F,
                                                const double* __restrict__ rz, double K, Cosmo C,
                                                double* __restrict__ h2fdf) {
-    // outputs are (m, q, z, f) with f fastest; a block of 256 spans at most two (m, q) bins when a
-    // bin holds at least 256 outputs, and then their mass factors are computed once, in LDS
-    __shared__ double s_g[2];
-    const int64_t per_mq = (int64_t)(Z - 1) * F;
-    const int64_t o0 = (int64_t)blockIdx.x * blockDim.x;
-    const int64_t mq0 = o0 / per_mq;
-    const int64_t nmq = (int64_t)(M - 1) * (Q - 1);
-    const bool shared_g = per_mq >= (int64_t)blockDim.x;
-    if (shared_g) {
-        if (threadIdx.x < 2 && mq0 + threadIdx.x < nmq) {
-            const int64_t mq = mq0 + threadIdx.x;
-            s_g[threadIdx.x] = h2_mass_factor(mtc, mrc, (int)(mq / (Q - 1)), (int)(mq % (Q - 1)), K);
-        }
-        __syncthreads();
-    }
-    int64_t o = o0 + threadIdx.x;
-    int64_t nout = nmq * (Z - 1) * F;
-    if (o >= nout) return;
-    int ff = (int)(o % F);
-    int64_t t = o / F;
-    int zz = (int)(t % (Z - 1));
-    t /= (Z - 1);
-    int qq = (int)(t % (Q - 1));
-    int mm = (int)(t / (Q - 1));
+    const uint32_t per_mq = (uint32_t)(Z - 1) * (uint32_t)F;
+    const uint32_t nchunk = (per_mq + blockDim.x - 1) / blockDim.x;
+    const int mq = (int)(blockIdx.x / nchunk);   // uniform: the block's chunks of one bin are consecutive
+    const int mm = mq / (Q - 1), qq = mq - mm * (Q - 1);
+    const uint32_t loc = (blockIdx.x - (uint32_t)mq * nchunk) * blockDim.x + threadIdx.x;
+    // the bin's mass factor (two powers and a cube root) once per block, not once per wave
+    __shared__ double s_g;
+    if (threadIdx.x == 0) s_g = h2_mass_factor(mtc, mrc, mm, qq, K);
+    __syncthreads();
+    if (loc >= per_mq) return;
+    const int zz = (int)(loc / (uint32_t)F), ff = (int)(loc - (uint32_t)zz * (uint32_t)F);
+    const int64_t o = (int64_t)mq * per_mq + loc;
     auto R = [&](int i, int j, int k) { return rz[((((int64_t)(mm + i) * Q) + (qq + j)) * Z + (zz + k)) * F + ff]; };
     // axis 0 (mass), then axis 1 (ratio), then axis 2 (redshift)
     double a00 = 0.5 * (R(0, 0, 0) + R(1, 0, 0));
@@ -61,7 +51,7 @@ __global__ void __launch_bounds__(256) k_h2fdf(const double* __restrict__ mtc, i
     double b1 = 0.5 * (a01 + a11);
     double zmid = 0.5 * (b0 + b1);
 
-    const double g = shared_g ? s_g[t - mq0] : h2_mass_factor(mtc, mrc, mm, qq, K);
+    const double g = s_g;
     double dc = (zmid > 0.0) ? cosmo_dcom(C, zmid) : __builtin_inf();
     double fr = fc[ff] * (1.0 + zmid);
     double hs = g * cbrt(fr * fr) / dc;   // K M_c (2 M_c f_r)^(2/3) / d_c
@@ -129,9 +119,12 @@ extern "C" int hd_h2fdf_redz(const double* mt_cent, int M, const double* mr_cent
                              double* h2fdf, void* stream) {
     if (M < 2 || Q < 2 || Z < 2 || F < 1 || !dc_tab || !dc_der || n_tab < 8)
         return set_error(HD_ERR_ARG, "hd_h2fdf_redz: bad arguments");
-    int64_t n = (int64_t)(M - 1) * (Q - 1) * (Z - 1) * F;
+    const int64_t per_mq = (int64_t)(Z - 1) * F;
+    const int64_t nblk = (int64_t)(M - 1) * (Q - 1) * cdiv(per_mq, 256);
+    if (per_mq > 0xffffffffll || nblk > 0x7fffffffll) return set_error(HD_ERR_SIZE, "hd_h2fdf_redz: grid too large");
     Cosmo C = make_cosmo(om0, h, dc_tab, dc_der, n_tab);
-    hipLaunchKernelGGL(k_h2fdf, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, mt_cent, M, mr_cent, Q,
+    hipLaunchKernelGGL(k_h2fdf, dim3((unsigned)nblk), dim3(256), 0,
+                       (hipStream_t)stream, mt_cent, M, mr_cent, Q,
                        Z, fc, fc_over_df, F, redz_final, gw_src_const, C, h2fdf);
     return check_launch("k_h2fdf");
 }
