@@ -25,6 +25,23 @@ namespace hd {
 // bracketing algorithm used by scipy's brentq, which the reference calls with
 // xtol=1e-3, rtol=1e-5, maxiter=100 on [-20, 20] (sam_cyutils.pyx:332-349).
 // ---------------------------------------------------------------------------
+// x^e for x > 0: from products and at most one square root when 2e is an integer of magnitude
+// up to 32 (the hardening model's exponents, -3.5 and -2 by default; a few ulp), pow otherwise.
+// e is the same for every thread, so the branch is uniform.
+__device__ __forceinline__ double pow_halfint(double x, double e) {
+    const double a = fabs(e);
+    if (!(a <= 16.0) || 2.0 * a != floor(2.0 * a)) return pow(x, e);
+    int n = (int)a;
+    double r = (a - (double)n != 0.0) ? sqrt(x) : 1.0;
+    double b = x;
+    while (n > 0) {
+        if (n & 1) r *= b;
+        b *= b;
+        n >>= 1;
+    }
+    return (e < 0.0) ? 1.0 / r : r;
+}
+
 struct LifeArgs {
     double mt, mr, target, rchar, g_in, g_out;
     double x_lane0;   // sepa_log10 before this lane's first step
@@ -492,6 +509,7 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
         }
     }
     __syncthreads();   // every recorded step is in place
+    const double mt13 = pow(mt, 1.0 / 3.0);
     // phase 2: the per-target work for every recorded (kk, ff)
     for (int p = tid; p < nzf; p += blockDim.x) {
         const int step = RF_LDS ? s_rf[p] : (int)((long long)rfg[p] - 1);
@@ -520,8 +538,13 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
         const double dcom = interp_at_index(inew, new_time, s_gage, s_gdc);
         rf0[p] = new_redz;
         const double frst = ft * (1.0 + new_redz);
-        const double sepa = kepler_sepa_from_freq(K, mt, frst);
-        const double dadt = hard_func_2pwl_gw(K, mt, mr, sepa, norm, rchar, g_in, g_out);
+        // Kepler separation and the 2PL + GW rate at the target (sam_cyutils.pyx:240-253, 52-61)
+        // with the bin-independent factor hoisted and cheaper powers: f^(2/3) as cbrt(f^2), the
+        // power laws of half-integer exponents from products and one square root (a few ulp)
+        const double sepa = K.kepler_sepa * mt13 / cbrt(frst * frst);
+        const double xx = sepa / rchar;
+        const double dadt = -norm * pow_halfint(1.0 + xx, g_in - g_out) / pow_halfint(xx, g_in - 1.0) +
+                            hard_gw(K, mt, mr, sepa);
         const double tres = -(2.0 / 3.0) * sepa / dadt;
         const double dmpc = dcom / MY_MPC;
         const double cosmo_fact = K.four_pi_c_mpc * (1.0 + new_redz) * (dmpc * dmpc);
