@@ -404,9 +404,10 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
     for (int s = tid + 1; s <= nsteps; s += blockDim.x)
         s_dt[s] = 2.0 * (s_sepa[s] - s_sepa[s - 1]) / (s_dadt[s - 1] + s_dadt[s]);
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 0) {   // the reference's sequential sum; unrolled so that the loads run ahead of it
         double t = 0.0;
         s_tevo[0] = 0.0;
+#pragma unroll 8
         for (int s = 1; s <= nsteps; s++) {
             t += s_dt[s];
             s_tevo[s] = t;
