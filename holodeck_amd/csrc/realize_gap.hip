@@ -615,11 +615,12 @@ struct GzMax { __device__ double operator()(double a, double b) const { return f
 // with the CDF summed from kb.  The counts kb .. kt + n - 1 are split into
 // contiguous segments of the group's lanes; p(kb) = p0 comes from the prep kernel
 // and the pmf follows p(k+1) = p(k) lam / (k+1).  Pass 1: each lane's pmf relative
-// to its segment's first count, their product and their sum; a group prefix product
-// gives each segment's first pmf, a group prefix sum of the segment sums the
-// carries.  Pass 2 repeats the relative sums and writes carry + first pmf * relative
-// sum (one rounding, fma) for the counts from kt on (lanes wholly below kt write
-// nothing: the lower tail is only summed).  (Relative error of the CDF about
+// to its segment's first count, their product and their sum (the running relative
+// sums at the table's counts parked in the table's own words); a group prefix
+// product gives each segment's first pmf, a group prefix sum of the segment sums
+// the carries.  Pass 2 writes carry + first pmf * relative sum (one rounding, fma)
+// for the counts from kt on (lanes wholly below kt write nothing: the lower tail is
+// only summed).  (Relative error of the CDF about
 // (kt + n - kb) ulp, then rounding down to a multiple of 2^-64.)  Four records per
 // wave share the fixed cost of the scans; long tables take a whole wave, which
 // shortens each lane's serial recurrence (the builds of a batch are its critical
@@ -635,9 +636,16 @@ __device__ __noinline__ void gz_build(gz_lds_u32* hi, double lam, int kb, int kt
     const int j0 = gl * s;
     const int m = min(max(nb - j0, 0), s);
     const double kf = (double)(kb + j0);
-    double prod = 1.0, rel = 0.0;        // pass 1
+    const int jt = kt - kb - j0;         // segment index of the table's first count
+    // pass 1; the relative sum at every count of the table is parked in that entry's own two
+    // words (low half in the high-word slot, high half in the low-word slot) for pass 2
+    double prod = 1.0, rel = 0.0;
     for (int j = 0; j < m; j++) {
         rel += prod;
+        if (j >= jt) {
+            hi[j0 + j - (kt - kb)] = (uint32_t)__double2loint(rel);
+            lo[j0 + j - (kt - kb)] = (uint32_t)__double2hiint(rel);
+        }
         prod *= lam * gz_rcp(kf + (double)(j + 1));
     }
     const double pre = gz_group_prev<G>(gz_group_scan<G>(prod, 1.0, GzMul()), 1.0);   // over the lanes before
@@ -651,21 +659,15 @@ __device__ __noinline__ void gz_build(gz_lds_u32* hi, double lam, int kb, int kt
     // non-decreasing (so the inversion is a monotone map of the uniform).
     const double ninf = -__builtin_inf();
     const double before = gz_group_prev<G>(gz_group_scan<G>(last, ninf, GzMax()), ninf);
-    // pass 2, for segments that reach the table; the same relative sums and the same fma,
-    // so a segment's last value is exactly `last`
-    const int jt = kt - kb - j0;         // segment index of the table's first count
-    if (jt < m) {
-        double q = 1.0, r = 0.0;
-        for (int j = 0; j < m; j++) {
-            r += q;
-            if (j >= jt) {
-                uint32_t h, l;
-                gz_fixed64(fmax(__fma_rn(pstart, r, carry), before), h, l);
-                hi[j0 + j - (kt - kb)] = h;
-                lo[j0 + j - (kt - kb)] = l;
-            }
-            q *= lam * gz_rcp(kf + (double)(j + 1));
-        }
+    // pass 2, for the counts of the table: the parked relative sums (each lane reads back only
+    // what it wrote) and the same fma, so a segment's last value is exactly `last`
+    for (int j = max(jt, 0); j < m; j++) {
+        const int e = j0 + j - (kt - kb);
+        const double r = __hiloint2double((int)lo[e], (int)hi[e]);
+        uint32_t h, l;
+        gz_fixed64(fmax(__fma_rn(pstart, r, carry), before), h, l);
+        hi[e] = h;
+        lo[e] = l;
     }
 }
 
