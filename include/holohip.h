@@ -258,7 +258,8 @@ int hd_realize_record_counts(const double *number, const double *h2fdf, const in
  * streams and outputs for the same seed.  Its scratch is a fixed device array of
  * HD_NOALLOC_SCRATCH_BYTES on each GPU the library runs on, held until the process ends, so it
  * takes only sizes with hd_realize_workspace_size(B, F, R, L, npar) <= HD_NOALLOC_SCRATCH_BYTES
- * (about 3.5 M bin-frequency cells without parameters, e.g. 120,000 bins x 20 frequencies);
+ * (without parameters about 2.7 M bin-frequency cells at R = 500, L = 10 and 3.4 M at R = 100,
+ * L = 5, e.g. 120,000 bins x 20 frequencies x 100 realizations);
  * larger ones return HD_ERR_SIZE.  Every call on one GPU uses that GPU's one array: calls that
  * overlap in time on one GPU (other streams, other threads) race, and the caller must
  * serialise them.  Allocates nothing (no hipMalloc, hipFree or allocating library call).  The
