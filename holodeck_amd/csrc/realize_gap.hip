@@ -42,14 +42,17 @@
 // tile's records take its words in order (gz_table_draw32), so results do not
 // depend on launch shape or batching.
 //
-// Three kernels:
-//   k_gz_prep  -- per (tile of 512 bins, frequency): classify, compact into
-//                 per-tile dense records (lambda, h, key, CDF window and its
-//                 offset in the tile's concatenated tables) and sparse records
-//                 (cumulative lambda);
-//   k_gz_draw  -- per (chunk of tiles, frequency) workgroup, one realization
+// Four kernels:
+//   k_gz_prep  -- per (tile of 512 bins, group of 8 frequencies): classify,
+//                 compact into per-tile dense records (lambda, h, key, CDF
+//                 window and its offset in the tile's concatenated tables) and
+//                 sparse records (cumulative lambda); the tile's cost estimate;
+//   k_gz_plan  -- one workgroup: the draw workgroups' dispatch order, (chunk,
+//                 frequency) items by decreasing estimated cost;
+//   k_gz_draw  -- per (chunk of 8 tiles, frequency) workgroup, one realization
 //                 per thread: tables of consecutive records are built in LDS
-//                 in batches that fit, one wave per record; then every
+//                 in batches that fit (a lane per short table, a 16-lane row
+//                 per mid-size table, a wave per long one); then every
 //                 realization draws from them;
 //   k_gz_big   -- per (chunk, frequency), only where the chunk has records
 //                 without a table: reloads the partials, draws those records,
