@@ -65,6 +65,53 @@ def test_stageA_parity(which, gpu, oracle_lib):
     _close(h2fdf, o['h2fdf'].reshape(-1, F), 1e-9, what="h2fdf")
 
 
+@pytest.mark.parametrize("i0", [0, 44, 88])
+def test_stageA_full_size_mass_rows(i0, gpu, oracle_lib):
+    """The full-size grid in one run (91 x 81 x 101 edges, 40 frequencies, 2PL evolution; no mass
+    scatter, whose kernels have tests of their own) against the oracle on three consecutive
+    total-mass rows, with every ratio, redshift and frequency: density, Brent norms, redshift at
+    the targets per corner, number and strain in the two mass bins between the rows.  The
+    oracle's evolution is independent per (M, Q), so the rows are run alone."""
+    print(f"[stageA 91x81x101 F=40 mass rows {i0}..{i0 + 2}]", flush=True)
+    from oracle import pipeline as P
+    from oracle.cfg import oracle_cfg
+    from holodeck_amd import device as D
+    spec = configs.spec_config2(nfreq=40)
+    sam = configs.make_sam(spec, scatter=0.0)
+    hard = configs.make_hard(spec, sam)
+    number, h2fdf, rf = sam._number_h2fdf_device(configs.fobs_gw_edges(spec), hard)
+    cfg = oracle_cfg(spec)
+    cfg['mmbulge']['scatter_dex'] = 0.0
+    c = cfg['cosmo']
+    cosmo = P.OracleCosmo(h=c['h'], Om0=c['Om0'], size=c['size'], zmax=c['zmax'])
+    mtot, mrat, redz = P.edges_from_cfg(cfg)
+    M, Q, Z = len(mtot), len(mrat), len(redz)
+    assert (M, Q, Z) == (91, 81, 101)
+    dens, zprime, gmt = P.static_density(mtot, mrat, redz, cfg, cosmo)
+    dens = np.array(dens)
+    dens[zprime < 0.0] = 0.0
+    _close(sam.static_binary_density, dens, 1e-12, what="density")
+    rows = slice(i0, i0 + 3)
+    hd = cfg['hard']
+    mt2, mr2 = np.meshgrid(mtot[rows], mrat, indexing='ij')
+    nl = P.find_2pwl_norm(hd['time'], mt2.ravel(), mr2.ravel(), hd['sepa_init'], hd['rchar'], hd['gamma_inner'],
+                          hd['gamma_outer'], hd['num_steps']).reshape(mt2.shape)
+    _close(D.to_host(hard._norm_log10_dev).reshape(M, Q)[rows], nl, 1e-12, what="norm_log10")
+    fge = np.asarray(cfg['fobs_gw_edges'])
+    foe, foc = fge / 2.0, P.midpoints(fge) / 2.0
+    rf_o, dn_o = P.dbn_2pwl(foc, hd['sepa_init'], hd['num_steps'], 10.0 ** nl, hd['rchar'], hd['gamma_inner'],
+                            hd['gamma_outer'], dens[rows], mtot[rows], mrat, redz, gmt[rows], cosmo)
+    F = len(foc)
+    _close(D.to_host(rf).reshape(M, Q, Z, F)[rows], rf_o, 1e-9, what="redz_final")
+    edges = [mtot[rows], mrat, redz, foe]
+    n_o = P.integrate_3dx1d(edges, dn_o)
+    h_o = P.h2fdf_redz(edges, rf_o, cosmo)
+    bins = slice(i0, i0 + 2)
+    _close(D.to_host(number).reshape(M - 1, Q - 1, Z - 1, F)[bins], n_o, 1e-9, atol=1e-12 * np.abs(n_o).max(),
+           what="number")
+    _close(D.to_host(h2fdf).reshape(M - 1, Q - 1, Z - 1, F)[bins], h_o, 1e-9, what="h2fdf")
+
+
 def _rand_counts(number, R, seed):
     rng = np.random.default_rng(seed)
     lam = np.minimum(number, 1e6)
