@@ -323,7 +323,7 @@ __device__ __forceinline__ int bracket_increasing(int n, double val, const doubl
 constexpr int DBN_FOBS_LDS = 1024;   // target frequencies staged in LDS up to this many
 
 template <bool RF_LDS>
-__global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const double* __restrict__ fobs, int F,
+__global__ void __launch_bounds__(DBN_THREADS, 5) k_dbn_2pwl(EvoConsts K, const double* __restrict__ fobs, int F,
                                                           int sorted_f, double sepa_init, int nsteps,
                                                           const double* __restrict__ hard_norm, double rchar,
                                                           double g_in, double g_out, const double* __restrict__ nden,
@@ -339,14 +339,16 @@ __global__ void __launch_bounds__(DBN_THREADS) k_dbn_2pwl(EvoConsts K, const dou
     double* s_frst = dbn_lds;
     double* s_dt = s_frst + ns1;
     double* s_tevo = s_dt + ns1;
-    double* s_sepa = s_tevo + ns1;     // set-up only
-    double* s_dadt = s_sepa + ns1;     // set-up only
-    double* s_gz = s_dadt + ns1;
+    double* s_gz = s_tevo + ns1;
     double* s_gdc = s_gz + n_interp;
     double* s_gage = s_gdc + n_interp;
     double* s_agek = s_gage + n_interp;   // (Z)
     double* s_fobs = s_agek + Z;          // (F) when F <= DBN_FOBS_LDS
-    int* s_rf = (int*)(s_fobs + (F <= DBN_FOBS_LDS ? F : 0));   // (Z, F) recorded steps when RF_LDS
+    // the set-up's separations and rates, then (their last reader is past a barrier before the
+    // recorded steps are cleared) the (Z, F) recorded steps when RF_LDS, in the same space
+    double* s_sepa = s_fobs + (F <= DBN_FOBS_LDS ? F : 0);
+    double* s_dadt = s_sepa + ns1;
+    int* s_rf = (int*)s_sepa;
 
     const int ii = blockIdx.x / Q;
     const int jj = blockIdx.x % Q;
@@ -640,7 +642,9 @@ extern "C" int hd_dbn_2pwl(const double* fobs_orb, int F, int fobs_sorted, doubl
                         (F <= DBN_FOBS_LDS ? (size_t)F : 0)) * sizeof(double);
     if (lds > 65536)
         return set_error(HD_ERR_SIZE, "hd_dbn_2pwl: step/cosmology tables need %zu B of LDS (> 64 KiB)", lds);
-    const size_t lds_rf = lds + (size_t)Z * F * sizeof(int);   // with the recorded steps
+    // with the recorded steps, which share the set-up's separations and rates (2 (num_steps + 1) doubles)
+    const size_t lds_rf = lds - (size_t)2 * (num_steps + 1) * sizeof(double) +
+                          std::max((size_t)2 * (num_steps + 1) * sizeof(double), (size_t)Z * F * sizeof(int));
     if (lds_rf <= 65536)
         hipLaunchKernelGGL(k_dbn_2pwl<true>, dim3(M * Q), dim3(DBN_THREADS), lds_rf, (hipStream_t)stream, evo_consts(),
                            fobs_orb, F, fobs_sorted, sepa_init, num_steps, hard_norm, rchar, gamma_inner, gamma_outer,
