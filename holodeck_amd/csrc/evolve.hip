@@ -80,17 +80,19 @@ __device__ double lifetime_wave(const EvoConsts& K, const LifeArgs& a, double no
 // separation, (1 + x)^(g_in - g_out), x^(g_in - 1) with x = a / r_char, the GW rate, and the
 // step's 2 (a_right - a_left).  The rate is then ((-norm) * p1) / p2 + gw, the same operations in
 // the same order as hard_func_2pwl_gw, so every lifetime is bit for bit the one computed from
-// scratch.  Up to LIFE_PTS - 1 steps per lane (nsteps <= 64 (LIFE_PTS - 1)).
-constexpr int LIFE_PTS = 9;
+// scratch.  PTS - 1 steps per lane at most (nsteps <= 64 (PTS - 1)); the arrays are sized to the
+// launch (6 points for up to 320 steps) so that they stay in few registers.
+template <int PTS>
 struct LifePre {
-    double p1[LIFE_PTS], p2[LIFE_PTS], gw[LIFE_PTS], num[LIFE_PTS];
+    double p1[PTS], p2[PTS], gw[PTS], num[PTS];
 };
 
-__device__ __forceinline__ void lifetime_pre(const EvoConsts& K, const LifeArgs& a, LifePre& P) {
+template <int PTS>
+__device__ __forceinline__ void lifetime_pre(const EvoConsts& K, const LifeArgs& a, LifePre<PTS>& P) {
     const int npts = (a.s_begin < a.s_end) ? a.s_end - a.s_begin + 1 : 0;
     double x = a.x_lane0, sepa_prev = 0.0;
 #pragma unroll
-    for (int i = 0; i < LIFE_PTS; i++) {
+    for (int i = 0; i < PTS; i++) {
         if (i > 0) x -= a.dx;
         const double sepa = pow(10.0, x);
         const double xx = sepa / a.rchar;
@@ -103,13 +105,14 @@ __device__ __forceinline__ void lifetime_pre(const EvoConsts& K, const LifeArgs&
     }
 }
 
-__device__ double lifetime_wave_pre(const LifeArgs& a, const LifePre& P, double norm_log10) {
+template <int PTS>
+__device__ double lifetime_wave_pre(const LifeArgs& a, const LifePre<PTS>& P, double norm_log10) {
     const double norm = pow(10.0, norm_log10);
     const int nst = (a.s_begin < a.s_end) ? a.s_end - a.s_begin : 0;
     double part = 0.0;
     double dadt_left = -norm * P.p1[0] / P.p2[0] + P.gw[0];
 #pragma unroll
-    for (int i = 1; i < LIFE_PTS; i++) {
+    for (int i = 1; i < PTS; i++) {
         const double dadt_right = -norm * P.p1[i] / P.p2[i] + P.gw[i];
         if (i <= nst) part += P.num[i] / (dadt_left + dadt_right);
         dadt_left = dadt_right;
@@ -117,7 +120,9 @@ __device__ double lifetime_wave_pre(const LifeArgs& a, const LifePre& P, double 
     return wave_sum(part) - a.target;
 }
 
-__global__ void __launch_bounds__(256) k_find_2pwl_norm(EvoConsts K, double target_time, const double* __restrict__ mtot,
+// PTS: step boundaries per lane whose factors are kept in registers (PTS - 1 steps), 0 for none
+template <int PTS>
+__global__ void __launch_bounds__(256, (PTS == 6) ? 4 : 1) k_find_2pwl_norm(EvoConsts K, double target_time, const double* __restrict__ mtot,
                                                         const double* __restrict__ mrat, int64_t n, double sepa_init,
                                                         double rchar, double g_in, double g_out, int nsteps,
                                                         double* __restrict__ norm_log10, double* __restrict__ norm_out) {
@@ -143,10 +148,12 @@ __global__ void __launch_bounds__(256) k_find_2pwl_norm(EvoConsts K, double targ
     a.x_lane0 = x;
 
     // the normalisation-free factors of every step boundary once (registers), when they fit
-    const bool pre = chunk <= LIFE_PTS - 1;
-    LifePre P;
-    if (pre) lifetime_pre(K, a, P);
-    auto life = [&](double v) { return pre ? lifetime_wave_pre(a, P, v) : lifetime_wave(K, a, v); };
+    LifePre<(PTS > 0) ? PTS : 1> P;
+    if constexpr (PTS > 0) lifetime_pre(K, a, P);
+    auto life = [&](double v) {
+        if constexpr (PTS > 0) return lifetime_wave_pre(a, P, v);
+        else return lifetime_wave(K, a, v);
+    };
 
     const double xtol = 1e-3, rtol = 1e-5;
     const int maxiter = 100;
@@ -600,8 +607,16 @@ extern "C" int hd_find_2pwl_norm(double target_time, const double* mtot, const d
     if (n == 0) return HD_OK;
     const int waves_per_block = 4;
     dim3 grid((unsigned)cdiv(n, waves_per_block));
-    hipLaunchKernelGGL(k_find_2pwl_norm, grid, dim3(64 * waves_per_block), 0, (hipStream_t)stream, evo_consts(),
-                       target_time, mtot, mrat, n, sepa_init, rchar, gamma_inner, gamma_outer, nsteps, norm_log10, norm);
+    const int chunk = (nsteps + 63) / 64;   // steps per lane
+    if (chunk <= 5)
+        hipLaunchKernelGGL(k_find_2pwl_norm<6>, grid, dim3(64 * waves_per_block), 0, (hipStream_t)stream, evo_consts(),
+                           target_time, mtot, mrat, n, sepa_init, rchar, gamma_inner, gamma_outer, nsteps, norm_log10, norm);
+    else if (chunk <= 8)
+        hipLaunchKernelGGL(k_find_2pwl_norm<9>, grid, dim3(64 * waves_per_block), 0, (hipStream_t)stream, evo_consts(),
+                           target_time, mtot, mrat, n, sepa_init, rchar, gamma_inner, gamma_outer, nsteps, norm_log10, norm);
+    else
+        hipLaunchKernelGGL(k_find_2pwl_norm<0>, grid, dim3(64 * waves_per_block), 0, (hipStream_t)stream, evo_consts(),
+                           target_time, mtot, mrat, n, sepa_init, rchar, gamma_inner, gamma_outer, nsteps, norm_log10, norm);
     return check_launch("k_find_2pwl_norm");
 }
 
