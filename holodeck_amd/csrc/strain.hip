@@ -34,28 +34,32 @@ __global__ void __launch_bounds__(256) k_h2fdf(const double* __restrict__ mtc, i
     const int mq = (int)(blockIdx.x / nchunk);   // uniform: the block's chunks of one bin are consecutive
     const int mm = mq / (Q - 1), qq = mq - mm * (Q - 1);
     const uint32_t loc = (blockIdx.x - (uint32_t)mq * nchunk) * blockDim.x + threadIdx.x;
-    // the bin's mass factor (two powers and a cube root) once per block, not once per wave
+    // the bin's mass factor (two powers and a cube root) once per block, not once per wave; the
+    // other waves meanwhile load their corners and evaluate d_c and the cube root, and wait for it
+    // only where it is used
     __shared__ double s_g;
     if (threadIdx.x == 0) s_g = h2_mass_factor(mtc, mrc, mm, qq, K);
-    __syncthreads();
-    if (loc >= per_mq) return;
+    const bool live = loc < per_mq;
     const int zz = (int)(loc / (uint32_t)F), ff = (int)(loc - (uint32_t)zz * (uint32_t)F);
-    const int64_t o = (int64_t)mq * per_mq + loc;
-    auto R = [&](int i, int j, int k) { return rz[((((int64_t)(mm + i) * Q) + (qq + j)) * Z + (zz + k)) * F + ff]; };
-    // axis 0 (mass), then axis 1 (ratio), then axis 2 (redshift)
-    double a00 = 0.5 * (R(0, 0, 0) + R(1, 0, 0));
-    double a01 = 0.5 * (R(0, 0, 1) + R(1, 0, 1));
-    double a10 = 0.5 * (R(0, 1, 0) + R(1, 1, 0));
-    double a11 = 0.5 * (R(0, 1, 1) + R(1, 1, 1));
-    double b0 = 0.5 * (a00 + a10);
-    double b1 = 0.5 * (a01 + a11);
-    double zmid = 0.5 * (b0 + b1);
-
-    const double g = s_g;
-    double dc = (zmid > 0.0) ? cosmo_dcom(C, zmid) : __builtin_inf();
-    double fr = fc[ff] * (1.0 + zmid);
-    double hs = g * cbrt(fr * fr) / dc;   // K M_c (2 M_c f_r)^(2/3) / d_c
-    h2fdf[o] = (hs * hs) * fcdf[ff];
+    double dc = 1.0, t = 0.0;
+    if (live) {
+        auto R = [&](int i, int j, int k) { return rz[((((int64_t)(mm + i) * Q) + (qq + j)) * Z + (zz + k)) * F + ff]; };
+        // axis 0 (mass), then axis 1 (ratio), then axis 2 (redshift)
+        double a00 = 0.5 * (R(0, 0, 0) + R(1, 0, 0));
+        double a01 = 0.5 * (R(0, 0, 1) + R(1, 0, 1));
+        double a10 = 0.5 * (R(0, 1, 0) + R(1, 1, 0));
+        double a11 = 0.5 * (R(0, 1, 1) + R(1, 1, 1));
+        double b0 = 0.5 * (a00 + a10);
+        double b1 = 0.5 * (a01 + a11);
+        double zmid = 0.5 * (b0 + b1);
+        dc = (zmid > 0.0) ? cosmo_dcom(C, zmid) : __builtin_inf();
+        const double fr = fc[ff] * (1.0 + zmid);
+        t = cbrt(fr * fr);
+    }
+    __syncthreads();
+    if (!live) return;
+    const double hs = s_g * t / dc;   // K M_c (2 M_c f_r)^(2/3) / d_c
+    h2fdf[(int64_t)mq * per_mq + loc] = (hs * hs) * fcdf[ff];
 }
 
 __global__ void __launch_bounds__(256) k_bin_params(const double* __restrict__ mtc, int M, const double* __restrict__ mrc,
